@@ -1,0 +1,386 @@
+/*
+ * rtg.h -- C ABI of the MI355X trace step of expenses/ray-tracing-gallery.
+ *
+ * The reference builds its acceleration structures and traces rays through
+ * the Vulkan KHR ray-tracing extensions, called from Rust through `ash`.
+ * This header is the drop-in boundary that replaces those calls:
+ *
+ *   vkGetAccelerationStructureBuildSizesKHR  src/util_structs.rs:197-206,321-330;
+ *                                            src/util_functions.rs:486-495
+ *   vkCmdBuildAccelerationStructuresKHR      src/util_structs.rs:269-274 (BUILD),
+ *                                            src/util_structs.rs:345-354 (UPDATE)
+ *   vkCmdCopyAccelerationStructureKHR        src/util_structs.rs:389-395 (CLONE)
+ *   vkGetBufferDeviceAddress (AS address)    src/gpu_structs.rs:50;
+ *                                            src/command_buffer_recording.rs:112
+ *   vkCmdTraceRaysKHR(raygen,miss,hit,W,H,1) src/command_buffer_recording.rs:116-126
+ *
+ * together with the seven shader stages that the trace call runs
+ * (src/main.rs:243-305): ray_generation, primary_ray_miss, shadow_ray_miss,
+ * closest_hit_portal (shaders/ray-tracing/src/lib.rs), closest_hit_textured,
+ * closest_hit_mirror and any_hit_alpha_clip (the GLSL files under shaders/).
+ *
+ * Conventions
+ *  - Every entry point returns RTG_OK (0) or a negative rtg_status; the
+ *    message of the last failure on the calling thread is rtg_last_error().
+ *  - Device pointers are plain 64-bit device addresses (hipMalloc memory).
+ *    Host arrays are borrowed for the duration of the call only.
+ *  - Work is stream ordered on the hipStream_t passed as `stream`
+ *    (NULL = the null stream). Build entry points may synchronise their
+ *    stream; refit, clone and trace never synchronise.
+ *  - A context is used by one host thread at a time, as the reference records
+ *    every command on its event-loop thread (src/main.rs:728).
+ *  - The library owns acceleration-structure storage and its scratch arena;
+ *    callers own vertex/index/texture/instance/output buffers.
+ */
+#ifndef RTG_H
+#define RTG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTG_ABI_VERSION 1u
+
+typedef enum rtg_status {
+    RTG_OK = 0,
+    RTG_ERROR_INVALID_ARGUMENT = -1,
+    RTG_ERROR_OUT_OF_MEMORY = -2,
+    RTG_ERROR_DEVICE = -3,
+    RTG_ERROR_LIMIT = -4,
+    RTG_ERROR_NOT_FOUND = -5,
+    RTG_ERROR_BUDGET = -6,
+    RTG_ERROR_STATE = -7
+} rtg_status;
+
+/* ------------------------------------------------------------------------ */
+/* ABI structs shared with the reference (layouts are fixed; see asserts).   */
+/* ------------------------------------------------------------------------ */
+
+/* vk::AccelerationStructureInstanceKHR twin, 64 bytes
+ * (src/gpu_structs.rs:17-25, ::new :27-53, transpose_matrix_for_instance
+ * :55-58). `transform` is the row-major 3x4 object-to-world matrix.
+ * custom_index_and_mask = custom_index (low 24 bits) | mask << 24
+ * (vk::Packed24_8, gpu_structs.rs:45).
+ * sbt_offset_and_flags = hit-group offset (low 24) | instance flags << 24
+ * (gpu_structs.rs:35-49; HitShader Textured=0, Mirror=1, Portal=2,
+ * src/main.rs:46-50).
+ * acceleration_structure_reference = the rtg_blas address. */
+typedef struct rtg_instance {
+    float transform[3][4];
+    uint32_t custom_index_and_mask;
+    uint32_t sbt_offset_and_flags;
+    uint64_t acceleration_structure_reference;
+} rtg_instance;
+
+#define RTG_INSTANCE_TRIANGLE_FACING_CULL_DISABLE 0x1u
+#define RTG_INSTANCE_TRIANGLE_FLIP_FACING 0x2u
+#define RTG_INSTANCE_FORCE_OPAQUE 0x4u
+#define RTG_INSTANCE_FORCE_NO_OPAQUE 0x8u
+
+/* Hit groups of the reference pipeline (src/main.rs:46-50, :289-305). */
+#define RTG_HIT_TEXTURED 0u
+#define RTG_HIT_MIRROR 1u
+#define RTG_HIT_PORTAL 2u
+#define RTG_HIT_GROUP_COUNT 3u
+
+/* shared-structs/src/lib.rs:7-19 (Uniforms), GLSL mirror
+ * shaders/hit_shader_common.glsl:3-13. Column-major 4x4 matrices. 176 B. */
+typedef struct rtg_uniforms {
+    float view_inverse[16];
+    float proj_inverse[16];
+    float sun_dir[3];
+    uint32_t _padding0;
+    float sun_radius;
+    uint32_t blue_noise_texture_index;
+    uint32_t ggx_lut_texture_index;
+    uint32_t frame_index;
+    uint8_t show_heatmap;
+    uint8_t _padding1[15];
+} rtg_uniforms;
+
+/* shared-structs/src/lib.rs:21-29 (ModelInfo). Device addresses of the
+ * model's shared vertex streams (vec3 positions/normals, vec2 uvs, tightly
+ * packed) and of its rtg_geometry_info table. 32 B. */
+typedef struct rtg_model_info {
+    uint64_t position_buffer_address;
+    uint64_t normal_buffer_address;
+    uint64_t uv_buffer_address;
+    uint64_t geometry_info_address;
+} rtg_model_info;
+
+/* shared-structs/src/lib.rs:31-47 (GeometryInfo + GeometryImages). The
+ * index buffer holds one uvec3 of u32 vertex indices per triangle. A
+ * negative normal_map_image_index means "no normal map". 24 B. */
+typedef struct rtg_geometry_info {
+    uint64_t index_buffer_address;
+    uint32_t diffuse_image_index;
+    uint32_t metallic_roughness_image_index;
+    int32_t normal_map_image_index;
+    uint32_t _padding;
+} rtg_geometry_info;
+
+/* ------------------------------------------------------------------------ */
+/* Textures: one entry of the bindless sampler2D array                      */
+/* (hit_shader_common.glsl:67; ImageManager src/util_structs.rs:1296-1377). */
+/* ------------------------------------------------------------------------ */
+#define RTG_FORMAT_RGBA8_UNORM 0u  /* R8G8B8A8_UNORM: blue noise, LUT, normal maps */
+#define RTG_FORMAT_RGBA8_SRGB 1u   /* R8G8B8A8_SRGB: diffuse and metallic-roughness */
+#define RTG_FORMAT_RGBA32_FLOAT 2u /* R32G32B32A32_SFLOAT: 1x1 fallback colours */
+
+#define RTG_FILTER_NEAREST 0u /* the default sampler (util_structs.rs:1311-1315) */
+#define RTG_FILTER_LINEAR 1u  /* glTF magFilter != NEAREST (util_structs.rs:954-955) */
+
+typedef struct rtg_texture {
+    uint64_t data;   /* device address of width*height texels, row-major, no padding */
+    uint32_t width;
+    uint32_t height;
+    uint32_t format; /* RTG_FORMAT_* */
+    uint32_t filter; /* RTG_FILTER_*; wrap is always REPEAT, LOD 0 */
+} rtg_texture;
+
+/* ------------------------------------------------------------------------ */
+/* Context                                                                  */
+/* ------------------------------------------------------------------------ */
+typedef struct rtg_ctx rtg_ctx;
+
+/* Creates a context on HIP device `device` (makes it current). */
+int rtg_create(rtg_ctx** out, int device);
+void rtg_destroy(rtg_ctx* ctx);
+/* Message of the last failing call on this thread ("" if none). */
+const char* rtg_last_error(void);
+/* RTG_ABI_VERSION of the built library. */
+uint32_t rtg_abi_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Bottom-level acceleration structures                                     */
+/* ≙ AccelerationStructure::build_blas (src/util_structs.rs:140-224)       */
+/* ------------------------------------------------------------------------ */
+
+/* One triangle geometry (vk::AccelerationStructureGeometryTrianglesDataKHR,
+ * util_structs.rs:160-176): R32G32B32_SFLOAT vertices with `vertex_stride`
+ * bytes between vertices, UINT32 indices (3 per triangle). */
+typedef struct rtg_geometry {
+    uint64_t vertex_data;   /* device address */
+    uint32_t vertex_stride; /* bytes, >= 12 and a multiple of 4 */
+    uint32_t max_vertex;    /* highest valid vertex index (num_vertices - 1, or 0) */
+    uint64_t index_data;    /* device address */
+    uint32_t index_count;   /* multiple of 3 */
+    uint32_t flags;         /* RTG_GEOMETRY_OPAQUE */
+} rtg_geometry;
+
+#define RTG_GEOMETRY_OPAQUE 0x1u /* vk::GeometryFlagsKHR::OPAQUE (util_structs.rs:154-158) */
+
+/* Build flags (vk::BuildAccelerationStructureFlagsKHR). PREFER_FAST_TRACE is
+ * what the reference passes (util_structs.rs:194, util_functions.rs:480-483);
+ * ALLOW_UPDATE is accepted for TLASes, which can always be refitted. */
+#define RTG_BUILD_ALLOW_UPDATE 0x1u
+#define RTG_BUILD_PREFER_FAST_TRACE 0x4u
+
+typedef uint64_t rtg_blas; /* device address of the BLAS (instance.acceleration_structure_reference) */
+typedef uint64_t rtg_tlas; /* device address of the TLAS (PushConstantBufferAddresses.acceleration_structure) */
+
+/* ≙ vkGetAccelerationStructureBuildSizesKHR. */
+typedef struct rtg_build_sizes {
+    uint64_t acceleration_structure_size;
+    uint64_t build_scratch_size;
+    uint64_t update_scratch_size;
+} rtg_build_sizes;
+
+int rtg_blas_query_sizes(rtg_ctx* ctx, const rtg_geometry* geometries, uint32_t geometry_count,
+                         rtg_build_sizes* out);
+/* Builds a BLAS over `geometry_count` geometries (host array of descriptors
+ * whose buffers are on the device). Geometry index = position in the array,
+ * primitive index = triangle index within its geometry. */
+int rtg_blas_build(rtg_ctx* ctx, const rtg_geometry* geometries, uint32_t geometry_count,
+                   uint32_t build_flags, rtg_blas* out, void* stream);
+int rtg_blas_destroy(rtg_ctx* ctx, rtg_blas blas);
+
+/* ------------------------------------------------------------------------ */
+/* Top-level acceleration structures                                        */
+/* ≙ build_tlas (src/util_functions.rs:453-510),                            */
+/*   AccelerationStructure::update_tlas (src/util_structs.rs:285-357),      */
+/*   AccelerationStructure::clone (src/util_structs.rs:359-405)             */
+/* ------------------------------------------------------------------------ */
+
+/* `instances` is a device array of `count` rtg_instance records. */
+int rtg_tlas_query_sizes(rtg_ctx* ctx, uint32_t count, rtg_build_sizes* out);
+int rtg_tlas_build(rtg_ctx* ctx, const rtg_instance* instances, uint32_t count,
+                   uint32_t build_flags, rtg_tlas* out, void* stream);
+/* In-place UPDATE (src == dst, util_structs.rs:318-319): new transforms and
+ * BLAS references with the same instance count; topology is kept. One kernel. */
+int rtg_tlas_refit(rtg_ctx* ctx, rtg_tlas tlas, const rtg_instance* instances, uint32_t count,
+                   void* stream);
+/* CLONE mode copy into a new TLAS (src/main.rs:622). */
+int rtg_tlas_clone(rtg_ctx* ctx, rtg_tlas src, rtg_tlas* out, void* stream);
+int rtg_tlas_destroy(rtg_ctx* ctx, rtg_tlas tlas);
+
+/* Statistics of a built structure (host copy of its header; synchronises). */
+typedef struct rtg_as_info {
+    uint32_t kind;        /* 0 = BLAS, 1 = TLAS */
+    uint32_t node_count;  /* 64-byte nodes */
+    uint32_t prim_count;  /* triangles (BLAS) or instances (TLAS) */
+    uint32_t geometry_count;
+    uint32_t max_depth;   /* longest root-to-leaf path, in nodes */
+    uint32_t node_width;  /* children per node */
+    float bounds[6];      /* root box lo.xyz, hi.xyz */
+    uint64_t size_bytes;
+} rtg_as_info;
+int rtg_as_get_info(rtg_ctx* ctx, uint64_t as_address, rtg_as_info* out);
+
+/* ------------------------------------------------------------------------ */
+/* Ray queries (the traversal core without shading)                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct rtg_ray {
+    float origin[3];
+    float tmin;
+    float direction[3];
+    float tmax;
+} rtg_ray; /* 32 B */
+
+/* Closest-hit record. instance = gl_InstanceID (index into the TLAS's
+ * instance array), geometry = gl_GeometryIndexEXT, primitive = gl_PrimitiveID,
+ * (u, v) = hitAttributeEXT barycentrics of vertices 1 and 2, t = gl_HitTEXT.
+ * A miss has instance == RTG_NO_HIT and t == tmax of the ray. */
+typedef struct rtg_hit {
+    float t;
+    uint32_t instance;
+    uint32_t primitive;
+    uint32_t geometry;
+    float u;
+    float v;
+} rtg_hit; /* 24 B */
+
+#define RTG_NO_HIT 0xFFFFFFFFu
+
+/* gl_RayFlags*EXT that the traversal honours. */
+#define RTG_RAY_FLAG_NONE 0x00u
+#define RTG_RAY_FLAG_OPAQUE 0x01u
+#define RTG_RAY_FLAG_NO_OPAQUE 0x02u
+#define RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT 0x04u
+#define RTG_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER 0x08u
+
+/* Scene bindings of a trace: ≙ the push constants
+ * (PushConstantBufferAddresses, shared-structs/src/lib.rs:49-56) and the
+ * bindless texture descriptor set (src/main.rs:416-460). Needed by the
+ * any-hit stage (alpha clip) and by the hit shaders. */
+typedef struct rtg_scene_bindings {
+    uint64_t model_info;   /* device address of rtg_model_info[model_count] */
+    uint64_t textures;     /* device address of rtg_texture[texture_count] */
+    uint32_t model_count;
+    uint32_t texture_count;
+} rtg_scene_bindings;
+
+/* Traces `count` rays (device array) against `tlas`.
+ * mode RTG_QUERY_CLOSEST writes `hits[count]`; RTG_QUERY_OCCLUSION writes
+ * `occluded[count]` (1 = some accepted hit in [tmin, tmax], else 0).
+ * ray_flags as gl_RayFlags; cull_mask as traceRayEXT's cullMask;
+ * sbt_offset selects the hit group (instance offset + sbt_offset) whose
+ * any-hit stage (alpha clip on non-opaque geometry) runs. `scene` may be
+ * NULL when every geometry is opaque. */
+#define RTG_QUERY_CLOSEST 0u
+#define RTG_QUERY_OCCLUSION 1u
+int rtg_trace_rays(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindings* scene,
+                   const rtg_ray* rays, uint32_t count, uint32_t mode, uint32_t ray_flags,
+                   uint32_t cull_mask, uint32_t sbt_offset, rtg_hit* hits, uint32_t* occluded,
+                   void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* The trace step: ≙ vkCmdTraceRaysKHR(W, H, 1) with the reference's        */
+/* seven stages (PerFrameResources::record,                                 */
+/* src/command_buffer_recording.rs:79-126).                                  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct rtg_trace_opts {
+    uint32_t shadow_rays; /* cone shadow rays per textured hit; reference: 2 (closest_hit_textured.glsl:195) */
+    uint32_t max_traces;  /* traces per path; reference: 3 (shaders/ray-tracing/src/lib.rs:144) */
+    uint32_t spp;         /* samples per pixel; 1 = pixel centre as in lib.rs:126 */
+    uint32_t flags;       /* RTG_TRACE_* */
+} rtg_trace_opts;
+
+/* Heatmap mode: colour = heatmap_temperature(cycles / 1e6) (lib.rs:174-186);
+ * also enabled by uniforms.show_heatmap. */
+#define RTG_TRACE_HEATMAP 0x1u
+
+typedef struct rtg_image_job {
+    rtg_tlas tlas;
+    rtg_scene_bindings scene;
+    rtg_uniforms uniforms;
+    uint32_t width;
+    uint32_t height;
+    rtg_trace_opts opts;
+    uint32_t* out_rgba8;       /* device, width*height packed RGBA8 (sRGB-encoded, A = 255); may be NULL */
+    float* out_rgb;            /* device, width*height*3 linear colour before encoding; may be NULL */
+    rtg_hit* out_primary_hits; /* device, width*height*spp first-trace hits; may be NULL */
+} rtg_image_job;
+
+/* One frame of one image (the plain single-request path). */
+int rtg_trace_image(rtg_ctx* ctx, const rtg_image_job* job, void* stream);
+/* Many images of uneven size (different scenes, resolutions, spp) in one set
+ * of persistent launches. Results equal tracing each job alone. */
+int rtg_trace_batch(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t job_count, void* stream);
+
+/* Ray and traversal counters of the last rtg_trace_image / rtg_trace_batch /
+ * rtg_trace_rays call made with counters enabled (synchronises). */
+typedef struct rtg_trace_counters {
+    uint64_t primary_rays;     /* first-trace rays */
+    uint64_t bounce_rays;      /* mirror/portal continuation rays */
+    uint64_t shadow_rays;      /* occlusion rays */
+    /* closest-hit traversals */
+    uint64_t node_visits;      /* 64-byte node records fetched */
+    uint64_t triangle_tests;   /* triangle records tested */
+    uint64_t instance_visits;  /* TLAS leaf -> BLAS transitions */
+    uint64_t anyhit_calls;     /* alpha-clip evaluations */
+    /* occlusion traversals */
+    uint64_t occ_node_visits;
+    uint64_t occ_triangle_tests;
+    uint64_t occ_instance_visits;
+    uint64_t occ_anyhit_calls;
+} rtg_trace_counters;
+/* Enables traversal counters (slower kernels) for subsequent traces. */
+int rtg_set_counters(rtg_ctx* ctx, int enable);
+int rtg_get_counters(rtg_ctx* ctx, rtg_trace_counters* out);
+
+/* Per-kernel-class GPU time (HIP events on the launch stream), accumulated
+ * while timing is enabled. kind: RTG_KERNEL_*; returns total ms and launches. */
+#define RTG_KERNEL_RAYGEN 0u
+#define RTG_KERNEL_TRACE_CLOSEST 1u
+#define RTG_KERNEL_SHADE 2u
+#define RTG_KERNEL_TRACE_OCCLUSION 3u
+#define RTG_KERNEL_RESOLVE 4u
+#define RTG_KERNEL_REFIT 5u
+#define RTG_KERNEL_COUNT 6u
+int rtg_set_timing(rtg_ctx* ctx, int enable);
+int rtg_get_kernel_time(rtg_ctx* ctx, uint32_t kind, double* total_ms, uint64_t* launches);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#ifdef __cplusplus
+static_assert(sizeof(rtg_instance) == 64, "instance record is 64 bytes");
+static_assert(offsetof(rtg_instance, custom_index_and_mask) == 48, "instance layout");
+static_assert(offsetof(rtg_instance, acceleration_structure_reference) == 56, "instance layout");
+static_assert(sizeof(rtg_uniforms) == 176, "Uniforms is 176 bytes");
+static_assert(offsetof(rtg_uniforms, sun_dir) == 128, "Uniforms layout");
+static_assert(offsetof(rtg_uniforms, sun_radius) == 144, "Uniforms layout");
+static_assert(offsetof(rtg_uniforms, frame_index) == 156, "Uniforms layout");
+static_assert(offsetof(rtg_uniforms, show_heatmap) == 160, "Uniforms layout");
+static_assert(sizeof(rtg_model_info) == 32, "ModelInfo is 32 bytes");
+static_assert(sizeof(rtg_geometry_info) == 24, "GeometryInfo is 24 bytes");
+static_assert(sizeof(rtg_ray) == 32, "ray is 32 bytes");
+static_assert(sizeof(rtg_hit) == 24, "hit is 24 bytes");
+static_assert(sizeof(rtg_texture) == 24, "texture descriptor is 24 bytes");
+#else
+_Static_assert(sizeof(rtg_instance) == 64, "instance record is 64 bytes");
+_Static_assert(sizeof(rtg_uniforms) == 176, "Uniforms is 176 bytes");
+_Static_assert(sizeof(rtg_model_info) == 32, "ModelInfo is 32 bytes");
+_Static_assert(sizeof(rtg_geometry_info) == 24, "GeometryInfo is 24 bytes");
+_Static_assert(sizeof(rtg_ray) == 32, "ray is 32 bytes");
+_Static_assert(sizeof(rtg_hit) == 24, "hit is 24 bytes");
+#endif
+
+#endif /* RTG_H */

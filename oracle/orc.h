@@ -1,0 +1,92 @@
+/*
+ * orc.h -- the CPU oracle of the trace step (TEST INFRASTRUCTURE).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load liborc.so. The product never links, imports or calls it.
+ *
+ * The oracle restates, on the host, in plain C:
+ *   - the LBVH build of bottom- and top-level acceleration structures
+ *     (Karras 2012, "Maximizing Parallelism in the Construction of BVHs,
+ *     Octrees, and k-d Trees"), producing byte-identical blobs to the GPU
+ *     build (the address field of TLAS instance records aside);
+ *   - the TLAS refit (src/util_structs.rs:285-357 semantics: UPDATE in place);
+ *   - two-level traversal with the watertight ray/triangle test (Woop,
+ *     Benthin and Wald 2013, JCGT 2(1)) and the robust slab test (Ize 2013),
+ *     closest hit with a (t, instance, geometry, primitive) tie-break, and
+ *     any-hit occlusion, with any_hit_alpha_clip on non-opaque geometry
+ *     (shaders/any_hit_alpha_clip.glsl:11-29);
+ *   - the seven shader stages of the reference's pipeline
+ *     (src/main.rs:243-305) run per pixel: ray_generation
+ *     (shaders/ray-tracing/src/lib.rs:94-191), primary_ray_miss (:40-51),
+ *     shadow_ray_miss (:33-36), closest_hit_portal (:300-312),
+ *     closest_hit_textured (shaders/closest_hit_textured.glsl:174-226 with
+ *     shaders/pbr.glsl:174-211), closest_hit_mirror
+ *     (shaders/closest_hit_mirror.glsl:11-29).
+ *
+ * Device addresses in the ABI structs (rtg_model_info, rtg_geometry_info,
+ * rtg_texture, rtg_geometry, instance BLAS references) are host pointers here.
+ */
+#ifndef ORC_H
+#define ORC_H
+
+#include <stdint.h>
+#include "../include/rtg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Builds a BLAS blob in host memory (free with orc_free). */
+int orc_blas_build(const rtg_geometry* geoms, uint32_t geom_count, void** out_blob,
+                   uint64_t* out_size);
+/* Builds a TLAS blob; instance BLAS references are orc BLAS blob pointers. */
+int orc_tlas_build(const rtg_instance* instances, uint32_t count, void** out_blob,
+                   uint64_t* out_size);
+/* In-place refit of a TLAS blob with new instance records (same count). */
+int orc_tlas_refit(void* tlas_blob, const rtg_instance* instances, uint32_t count);
+void orc_free(void* blob);
+int orc_as_get_info(const void* blob, rtg_as_info* out);
+
+/* Ray queries (rtg_trace_rays semantics). threads <= 0 means all cores. */
+int orc_trace_rays(const void* tlas, const rtg_scene_bindings* scene, const rtg_ray* rays,
+                   uint32_t count, uint32_t mode, uint32_t ray_flags, uint32_t cull_mask,
+                   uint32_t sbt_offset, rtg_hit* hits, uint32_t* occluded,
+                   rtg_trace_counters* counters, int threads);
+/* The same queries by testing every triangle of every instance (no BVH). */
+int orc_trace_rays_brute(const void* tlas, const rtg_scene_bindings* scene, const rtg_ray* rays,
+                         uint32_t count, uint32_t mode, uint32_t ray_flags, uint32_t cull_mask,
+                         uint32_t sbt_offset, rtg_hit* hits, uint32_t* occluded);
+
+/* One frame of one image job (rtg_trace_image semantics). If pixel_list is
+ * non-NULL only those pixels (y * width + x) are rendered. threads <= 0 means
+ * all cores. */
+int orc_trace_image(const rtg_image_job* job, const uint32_t* pixel_list, uint32_t pixel_count,
+                    int threads, rtg_trace_counters* counters);
+
+/* Exposed shading helpers for known-answer tests. */
+float orc_v_smith_ggx_correlated(float NoV, float NoL, float roughness);
+float orc_d_ggx(float NoH, float roughness);
+void orc_brdf(const float normal[3], const float view[3], const float light[3],
+              float perceptual_roughness, const float base_colour[3], float metallic,
+              float reflectance, float light_intensity, float out[3]);
+void orc_linear_to_srgb(const float in[3], float out[3]);
+void orc_heatmap_temperature(float heat, float out[3]);
+void orc_sample_directional_light(const float rng[2], const float centre[3], float radius,
+                                  float out[3]);
+void orc_sample_texture(const rtg_texture* tex, const float srgb_lut[256], float u, float v,
+                        float out[4]);
+void orc_srgb_lut(float lut[256]);
+float orc_cos(float x);
+float orc_sin(float x);
+float orc_pow(float x, float y);
+float orc_log2(float x);
+float orc_exp2(float y);
+void orc_watertight(const float o[3], const float d[3], const float v0[3], const float v1[3],
+                    const float v2[3], float tmin, float tmax, int* hit, float* t, float* u,
+                    float* v);
+uint32_t orc_morton3(float cx, float cy, float cz, const float lo[3], const float hi[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

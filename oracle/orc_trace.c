@@ -1,0 +1,963 @@
+/*
+ * orc_trace.c -- CPU oracle: traversal, the reference's shader stages and
+ * the per-pixel trace step (TEST INFRASTRUCTURE; never part of the product).
+ *
+ * Stage semantics restated here, each from the file:line named at the
+ * function. Operation order follows the reference's source expressions,
+ * left to right, one rounding per written operation (no FMA).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#include "orc.h"
+#include "orc_layout.h"
+#include "orc_math.h"
+
+#define STACK_MAX 512u
+#define SENTINEL 0xFFFFFFFEu
+/* 1 + 2*gamma(3) rounded up: far-plane inflation of the robust slab test
+ * (Ize 2013, "Robust BVH Ray Traversal", JCGT 2(2)). */
+#define FAR_SCALE_BITS 0x3f800003u
+
+typedef struct {
+    float t, u, v;
+    uint32_t inst, prim, geom;
+} hitrec;
+
+typedef struct {
+    const orc_header* th;
+    const orc_node* tnodes;
+    const orc_inst* tinst;
+    const rtg_instance* tsrc;
+    const rtg_model_info* models;
+    uint32_t model_count;
+    const rtg_texture* textures;
+    uint32_t texture_count;
+    float lut[256];
+    uint32_t ray_flags, cull_mask, sbt_offset;
+    rtg_trace_counters cnt;
+} qctx;
+
+/* ---------------------------------------------------------------- */
+/* Textures: the sampler2D lookups of the hit shaders (LOD 0, REPEAT). */
+/* ---------------------------------------------------------------- */
+
+void orc_srgb_lut(float lut[256]) {
+    /* IEC 61966-2-1 decode, evaluated in double, rounded once to float. */
+    for (int i = 0; i < 256; ++i) {
+        double c = i / 255.0;
+        double l = c <= 0.04045 ? c / 12.92 : pow((c + 0.055) / 1.055, 2.4);
+        lut[i] = (float)l;
+    }
+}
+
+static int wrap_coord(float f, uint32_t size) {
+    if (!(fabsf(f) < 16777216.0f)) return 0; /* non-finite or huge: texel 0 */
+    long long i = (long long)f;
+    long long m = i % (long long)size;
+    if (m < 0) m += size;
+    return (int)m;
+}
+
+static void fetch_texel(const rtg_texture* t, const float* lut, int x, int y, float out[4]) {
+    uint64_t idx = (uint64_t)y * t->width + (uint64_t)x;
+    if (t->format == RTG_FORMAT_RGBA32_FLOAT) {
+        const float* p = (const float*)(uintptr_t)t->data + 4 * idx;
+        out[0] = p[0]; out[1] = p[1]; out[2] = p[2]; out[3] = p[3];
+        return;
+    }
+    const uint8_t* p = (const uint8_t*)(uintptr_t)t->data + 4 * idx;
+    if (t->format == RTG_FORMAT_RGBA8_SRGB) {
+        out[0] = lut[p[0]]; out[1] = lut[p[1]]; out[2] = lut[p[2]];
+    } else {
+        out[0] = (float)p[0] / 255.0f; out[1] = (float)p[1] / 255.0f; out[2] = (float)p[2] / 255.0f;
+    }
+    out[3] = (float)p[3] / 255.0f;
+}
+
+/* texture(textures[i], uv) with the image's own sampler: NEAREST takes texel
+ * floor(uv * size); LINEAR weights the four texels around uv * size - 0.5. */
+static void sample_tex(const rtg_texture* t, const float* lut, float u, float v, float out[4]) {
+    if (t->width == 0 || t->height == 0) {
+        out[0] = out[1] = out[2] = 0.0f; out[3] = 1.0f;
+        return;
+    }
+    if (t->filter == RTG_FILTER_NEAREST) {
+        int x = wrap_coord(floorf(u * (float)t->width), t->width);
+        int y = wrap_coord(floorf(v * (float)t->height), t->height);
+        fetch_texel(t, lut, x, y, out);
+        return;
+    }
+    float x = u * (float)t->width - 0.5f;
+    float y = v * (float)t->height - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    float a = x - fx, b = y - fy;
+    int x0 = wrap_coord(fx, t->width), y0 = wrap_coord(fy, t->height);
+    int x1 = x0 + 1 == (int)t->width ? 0 : x0 + 1;
+    int y1 = y0 + 1 == (int)t->height ? 0 : y0 + 1;
+    float t00[4], t10[4], t01[4], t11[4];
+    fetch_texel(t, lut, x0, y0, t00);
+    fetch_texel(t, lut, x1, y0, t10);
+    fetch_texel(t, lut, x0, y1, t01);
+    fetch_texel(t, lut, x1, y1, t11);
+    float ia = 1.0f - a, ib = 1.0f - b;
+    for (int c = 0; c < 4; ++c) out[c] = (t00[c] * ia + t10[c] * a) * ib + (t01[c] * ia + t11[c] * a) * b;
+}
+
+static void sample_index(const qctx* q, uint32_t index, float u, float v, float out[4]) {
+    if (index >= q->texture_count) {
+        out[0] = out[1] = out[2] = 0.0f; out[3] = 1.0f;
+        return;
+    }
+    sample_tex(&q->textures[index], q->lut, u, v, out);
+}
+
+void orc_sample_texture(const rtg_texture* tex, const float lut[256], float u, float v, float out[4]) {
+    sample_tex(tex, lut, u, v, out);
+}
+
+/* ---------------------------------------------------------------- */
+/* Ray/triangle: watertight test (Woop, Benthin, Wald 2013, JCGT 2(1)) */
+/* ---------------------------------------------------------------- */
+
+typedef struct {
+    v3 o, d, inv;
+    int kx, ky, kz;
+    float sx, sy, sz;
+} oray;
+
+static void oray_setup(oray* r, v3 o, v3 d) {
+    r->o = o;
+    r->d = d;
+    r->inv = V3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    float dv[3] = {d.x, d.y, d.z};
+    if (dv[kz] < 0.0f) {
+        int tmp = kx;
+        kx = ky;
+        ky = tmp;
+    }
+    r->kx = kx;
+    r->ky = ky;
+    r->kz = kz;
+    r->sx = dv[kx] / dv[kz];
+    r->sy = dv[ky] / dv[kz];
+    r->sz = 1.0f / dv[kz];
+}
+
+/* Returns 1 on a hit with tmin <= t <= tmax; (u, v) weight vertices 1, 2. */
+static int tri_test(const oray* r, const float* p0, const float* p1, const float* p2, float tmin, float tmax,
+                    float* t_out, float* u_out, float* v_out) {
+    float A[3] = {p0[0] - r->o.x, p0[1] - r->o.y, p0[2] - r->o.z};
+    float B[3] = {p1[0] - r->o.x, p1[1] - r->o.y, p1[2] - r->o.z};
+    float C[3] = {p2[0] - r->o.x, p2[1] - r->o.y, p2[2] - r->o.z};
+    float Ax = A[r->kx] - r->sx * A[r->kz], Ay = A[r->ky] - r->sy * A[r->kz];
+    float Bx = B[r->kx] - r->sx * B[r->kz], By = B[r->ky] - r->sy * B[r->kz];
+    float Cx = C[r->kx] - r->sx * C[r->kz], Cy = C[r->ky] - r->sy * C[r->kz];
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return 0;
+    float det = (U + V) + W;
+    if (det == 0.0f) return 0;
+    float Az = r->sz * A[r->kz], Bz = r->sz * B[r->kz], Cz = r->sz * C[r->kz];
+    float T = (U * Az + V * Bz) + W * Cz;
+    float t = T / det;
+    if (!(t >= tmin && t <= tmax)) return 0;
+    *t_out = t;
+    *u_out = V / det;
+    *v_out = W / det;
+    return 1;
+}
+
+void orc_watertight(const float o[3], const float d[3], const float v0[3], const float v1[3], const float v2[3],
+                    float tmin, float tmax, int* hit, float* t, float* u, float* v) {
+    oray r;
+    oray_setup(&r, V3(o[0], o[1], o[2]), V3(d[0], d[1], d[2]));
+    *hit = tri_test(&r, v0, v1, v2, tmin, tmax, t, u, v);
+}
+
+/* Robust slab test of child s of a node; returns 1 on overlap of
+ * [tmin, tcur]; *tn receives the entry distance. */
+static int box_test(const orc_node* nd, int s, const oray* r, float tmin, float tcur, float* tn) {
+    /* near/far planes by the sign bit of the direction (-0 selects like < 0,
+       matching 1/-0 = -inf) */
+    float nx = signbit(r->d.x) ? nd->x[2 * s + 1] : nd->x[2 * s];
+    float fx = signbit(r->d.x) ? nd->x[2 * s] : nd->x[2 * s + 1];
+    float ny = signbit(r->d.y) ? nd->y[2 * s + 1] : nd->y[2 * s];
+    float fy = signbit(r->d.y) ? nd->y[2 * s] : nd->y[2 * s + 1];
+    float nz = signbit(r->d.z) ? nd->z[2 * s + 1] : nd->z[2 * s];
+    float fz = signbit(r->d.z) ? nd->z[2 * s] : nd->z[2 * s + 1];
+    float tnx = (nx - r->o.x) * r->inv.x, tfx = (fx - r->o.x) * r->inv.x;
+    float tny = (ny - r->o.y) * r->inv.y, tfy = (fy - r->o.y) * r->inv.y;
+    float tnz = (nz - r->o.z) * r->inv.z, tfz = (fz - r->o.z) * r->inv.z;
+    float tnear = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
+    float tfar = fminf(fminf(fminf(tfx, tfy), tfz) * u2f(FAR_SCALE_BITS), tcur);
+    *tn = tnear;
+    return tnear <= tfar;
+}
+
+static int is_opaque(uint32_t geom_word, uint32_t inst_flags, uint32_t ray_flags) {
+    int opaque = (geom_word & ORC_NONOPAQUE_BIT) == 0;
+    if (inst_flags & RTG_INSTANCE_FORCE_OPAQUE) opaque = 1;
+    else if (inst_flags & RTG_INSTANCE_FORCE_NO_OPAQUE) opaque = 0;
+    if (ray_flags & RTG_RAY_FLAG_OPAQUE) opaque = 1;
+    else if (ray_flags & RTG_RAY_FLAG_NO_OPAQUE) opaque = 0;
+    return opaque;
+}
+
+/* any_hit_alpha_clip (shaders/any_hit_alpha_clip.glsl:11-29): returns 1 if
+ * the candidate is kept (alpha >= 0.5). */
+static int alpha_clip_keep(qctx* q, uint64_t* anyhit_counter, uint32_t inst_index, uint32_t geom, uint32_t prim,
+                           float u, float v) {
+    ++*anyhit_counter;
+    if (!q->models) return 1;
+    uint32_t custom = q->tsrc[inst_index].custom_index_and_mask & 0xffffffu;
+    if (custom >= q->model_count) return 1;
+    const rtg_model_info* mi = &q->models[custom];
+    const rtg_geometry_info* gi = (const rtg_geometry_info*)(uintptr_t)mi->geometry_info_address + geom;
+    const uint32_t* idx = (const uint32_t*)(uintptr_t)gi->index_buffer_address + 3ull * prim;
+    const float* uvs = (const float*)(uintptr_t)mi->uv_buffer_address;
+    v3 bc = V3((1.0f - u) - v, u, v);
+    v2 a = {uvs[2 * idx[0]], uvs[2 * idx[0] + 1]};
+    v2 b = {uvs[2 * idx[1]], uvs[2 * idx[1] + 1]};
+    v2 c = {uvs[2 * idx[2]], uvs[2 * idx[2] + 1]};
+    v2 uv = v2_interp(a, b, c, bc);
+    float texel[4];
+    sample_index(q, gi->diffuse_image_index, uv.x, uv.y, texel);
+    return !(texel[3] < 0.5f);
+}
+
+static int key_less(uint32_t ia, uint32_t ga, uint32_t pa, uint32_t ib, uint32_t gb, uint32_t pb) {
+    if (ia != ib) return ia < ib;
+    if (ga != gb) return ga < gb;
+    return pa < pb;
+}
+
+/* Two-level traversal. Returns 1 if an accepted hit was found. */
+static int traverse(qctx* q, v3 o, v3 d, float tmin, float tmax, int occlusion, hitrec* best) {
+    uint32_t stack_ref[STACK_MAX];
+    float stack_tn[STACK_MAX];
+    uint32_t sp = 0;
+    oray world, obj;
+    oray_setup(&world, o, d);
+    const oray* cr = &world;
+    const orc_node* nodes = q->tnodes;
+    const orc_tri* tris = NULL;
+    const orc_inst* cur_inst = NULL;
+    int in_blas = 0;
+    float tcur = tmax;
+    int found = 0;
+    best->t = tmax;
+    best->u = best->v = 0.0f;
+    best->inst = best->prim = best->geom = RTG_NO_HIT;
+    /* counters of this query kind: nodes, triangles, instances, any-hits */
+    uint64_t* kc = occlusion ? &q->cnt.occ_node_visits : &q->cnt.node_visits;
+    uint32_t cur = 0;
+    int have = 1;
+    while (1) {
+        if (have) {
+            if (!(cur & ORC_LEAF_BIT)) {
+                const orc_node* nd = &nodes[cur];
+                kc[0]++;
+                float tn0 = 0.0f, tn1 = 0.0f;
+                int h0 = nd->child[0] != ORC_EMPTY && box_test(nd, 0, cr, tmin, tcur, &tn0);
+                int h1 = nd->child[1] != ORC_EMPTY && box_test(nd, 1, cr, tmin, tcur, &tn1);
+                if (h0 && h1) {
+                    int first = tn1 < tn0 ? 1 : 0;
+                    cur = nd->child[first];
+                    if (sp >= STACK_MAX) return -1;
+                    stack_ref[sp] = nd->child[1 - first];
+                    stack_tn[sp] = first ? tn0 : tn1;
+                    ++sp;
+                } else if (h0) {
+                    cur = nd->child[0];
+                } else if (h1) {
+                    cur = nd->child[1];
+                } else {
+                    have = 0;
+                }
+                continue;
+            }
+            if (!in_blas) {
+                const orc_inst* rec = &q->tinst[cur & 0x07ffffffu];
+                have = 0;
+                if (((rec->mask_flags & 0xffu) & q->cull_mask) == 0) continue;
+                const orc_header* bh = (const orc_header*)(uintptr_t)rec->blas;
+                if (!bh) continue;
+                kc[2]++;
+                if (sp >= STACK_MAX) return -1;
+                stack_ref[sp] = SENTINEL;
+                stack_tn[sp] = -INFINITY;
+                ++sp;
+                oray_setup(&obj, m34_point(rec->w2o, o), m34_dir(rec->w2o, d));
+                cr = &obj;
+                nodes = (const orc_node*)((const uint8_t*)bh + bh->nodes_off);
+                tris = (const orc_tri*)((const uint8_t*)bh + bh->prims_off);
+                cur_inst = rec;
+                in_blas = 1;
+                cur = 0;
+                have = 1;
+                continue;
+            }
+            uint32_t first = cur & 0x07ffffffu;
+            uint32_t count = ((cur >> 27) & 0xfu) + 1u;
+            for (uint32_t k = first; k < first + count; ++k) {
+                const orc_tri* tr = &tris[k];
+                kc[1]++;
+                float t, u, v;
+                if (!tri_test(cr, tr->v0, tr->v1, tr->v2, tmin, tcur, &t, &u, &v)) continue;
+                uint32_t geom = tr->geom & ~ORC_NONOPAQUE_BIT;
+                uint32_t inst = cur_inst->index;
+                if (!occlusion &&
+                    !(t < best->t || (t == best->t && key_less(inst, geom, tr->prim, best->inst, best->geom, best->prim))))
+                    continue;
+                uint32_t iflags = (cur_inst->mask_flags >> 8) & 0xffu;
+                if (!is_opaque(tr->geom, iflags, q->ray_flags)) {
+                    uint32_t group = (q->tsrc[inst].sbt_offset_and_flags & 0xffffffu) + q->sbt_offset;
+                    if (group < RTG_HIT_GROUP_COUNT && !alpha_clip_keep(q, &kc[3], inst, geom, tr->prim, u, v)) continue;
+                }
+                found = 1;
+                best->t = t;
+                best->u = u;
+                best->v = v;
+                best->inst = inst;
+                best->geom = geom;
+                best->prim = tr->prim;
+                if (occlusion || (q->ray_flags & RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT)) return 1;
+                tcur = t;
+            }
+            have = 0;
+            continue;
+        }
+        /* pop */
+        while (sp > 0) {
+            --sp;
+            uint32_t r = stack_ref[sp];
+            if (r == SENTINEL) {
+                in_blas = 0;
+                cr = &world;
+                nodes = q->tnodes;
+                continue;
+            }
+            if (stack_tn[sp] > tcur) continue;
+            cur = r;
+            have = 1;
+            break;
+        }
+        if (!have) break;
+    }
+    return found;
+}
+
+static int qctx_init(qctx* q, const void* tlas, const rtg_scene_bindings* scene) {
+    memset(q, 0, sizeof(*q));
+    const orc_header* h = (const orc_header*)tlas;
+    if (!h || h->magic != ORC_MAGIC || h->kind != ORC_KIND_TLAS) return RTG_ERROR_INVALID_ARGUMENT;
+    q->th = h;
+    q->tnodes = (const orc_node*)((const uint8_t*)tlas + h->nodes_off);
+    q->tinst = (const orc_inst*)((const uint8_t*)tlas + h->prims_off);
+    q->tsrc = (const rtg_instance*)((const uint8_t*)tlas + h->extra_off);
+    if (scene) {
+        q->models = (const rtg_model_info*)(uintptr_t)scene->model_info;
+        q->model_count = scene->model_count;
+        q->textures = (const rtg_texture*)(uintptr_t)scene->textures;
+        q->texture_count = scene->texture_count;
+    }
+    orc_srgb_lut(q->lut);
+    q->cull_mask = 0xffu;
+    return RTG_OK;
+}
+
+static void add_counters(rtg_trace_counters* dst, const rtg_trace_counters* s) {
+    dst->primary_rays += s->primary_rays;
+    dst->bounce_rays += s->bounce_rays;
+    dst->shadow_rays += s->shadow_rays;
+    dst->node_visits += s->node_visits;
+    dst->triangle_tests += s->triangle_tests;
+    dst->instance_visits += s->instance_visits;
+    dst->anyhit_calls += s->anyhit_calls;
+    dst->occ_node_visits += s->occ_node_visits;
+    dst->occ_triangle_tests += s->occ_triangle_tests;
+    dst->occ_instance_visits += s->occ_instance_visits;
+    dst->occ_anyhit_calls += s->occ_anyhit_calls;
+}
+
+/* ---------------------------------------------------------------- */
+/* Thread pool over work items                                       */
+/* ---------------------------------------------------------------- */
+typedef struct {
+    void (*fn)(void* arg, qctx* q, uint32_t item);
+    void* arg;
+    uint32_t items;
+    volatile uint32_t next;
+    pthread_mutex_t mu;
+    const void* tlas;
+    const rtg_scene_bindings* scene;
+    uint32_t ray_flags, cull_mask, sbt_offset;
+    rtg_trace_counters total;
+    int status;
+} pool;
+
+static void* pool_worker(void* p_) {
+    pool* p = (pool*)p_;
+    qctx* q = (qctx*)malloc(sizeof(qctx));
+    if (qctx_init(q, p->tlas, p->scene) != RTG_OK) {
+        free(q);
+        return NULL;
+    }
+    q->ray_flags = p->ray_flags;
+    q->cull_mask = p->cull_mask;
+    q->sbt_offset = p->sbt_offset;
+    while (1) {
+        uint32_t it = __sync_fetch_and_add(&p->next, 1u);
+        if (it >= p->items) break;
+        p->fn(p->arg, q, it);
+    }
+    pthread_mutex_lock(&p->mu);
+    add_counters(&p->total, &q->cnt);
+    pthread_mutex_unlock(&p->mu);
+    free(q);
+    return NULL;
+}
+
+static int run_pool(pool* p, int threads) {
+    if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+    if (threads < 1) threads = 1;
+    if ((uint32_t)threads > p->items) threads = p->items ? (int)p->items : 1;
+    pthread_mutex_init(&p->mu, NULL);
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * threads);
+    for (int i = 1; i < threads; ++i) pthread_create(&th[i], NULL, pool_worker, p);
+    pool_worker(p);
+    for (int i = 1; i < threads; ++i) pthread_join(th[i], NULL);
+    free(th);
+    pthread_mutex_destroy(&p->mu);
+    return RTG_OK;
+}
+
+/* ---------------------------------------------------------------- */
+/* Ray queries                                                       */
+/* ---------------------------------------------------------------- */
+typedef struct {
+    const rtg_ray* rays;
+    uint32_t count, mode;
+    rtg_hit* hits;
+    uint32_t* occluded;
+} rays_job;
+
+#define RAY_CHUNK 256u
+
+static void rays_item(void* a, qctx* q, uint32_t item) {
+    rays_job* j = (rays_job*)a;
+    uint32_t b = item * RAY_CHUNK, e = b + RAY_CHUNK < j->count ? b + RAY_CHUNK : j->count;
+    for (uint32_t i = b; i < e; ++i) {
+        const rtg_ray* r = &j->rays[i];
+        hitrec h;
+        int occl = j->mode == RTG_QUERY_OCCLUSION;
+        int f = traverse(q, V3(r->origin[0], r->origin[1], r->origin[2]),
+                         V3(r->direction[0], r->direction[1], r->direction[2]), r->tmin, r->tmax, occl, &h);
+        if (occl) {
+            j->occluded[i] = f > 0 ? 1u : 0u;
+        } else {
+            rtg_hit* o = &j->hits[i];
+            o->t = h.t;
+            o->instance = h.inst;
+            o->primitive = h.prim;
+            o->geometry = h.geom;
+            o->u = h.u;
+            o->v = h.v;
+        }
+    }
+}
+
+int orc_trace_rays(const void* tlas, const rtg_scene_bindings* scene, const rtg_ray* rays, uint32_t count,
+                   uint32_t mode, uint32_t ray_flags, uint32_t cull_mask, uint32_t sbt_offset, rtg_hit* hits,
+                   uint32_t* occluded, rtg_trace_counters* counters, int threads) {
+    if (mode > RTG_QUERY_OCCLUSION) return RTG_ERROR_INVALID_ARGUMENT;
+    if (mode == RTG_QUERY_CLOSEST && !hits) return RTG_ERROR_INVALID_ARGUMENT;
+    if (mode == RTG_QUERY_OCCLUSION && !occluded) return RTG_ERROR_INVALID_ARGUMENT;
+    qctx probe;
+    if (qctx_init(&probe, tlas, scene) != RTG_OK) return RTG_ERROR_INVALID_ARGUMENT;
+    rays_job j = {rays, count, mode, hits, occluded};
+    pool p;
+    memset(&p, 0, sizeof(p));
+    p.fn = rays_item;
+    p.arg = &j;
+    p.items = (count + RAY_CHUNK - 1) / RAY_CHUNK;
+    p.tlas = tlas;
+    p.scene = scene;
+    p.ray_flags = ray_flags;
+    p.cull_mask = cull_mask & 0xffu;
+    p.sbt_offset = sbt_offset;
+    run_pool(&p, threads);
+    if (counters) {
+        memset(counters, 0, sizeof(*counters));
+        add_counters(counters, &p.total);
+        counters->primary_rays = count;
+    }
+    return RTG_OK;
+}
+
+int orc_trace_rays_brute(const void* tlas, const rtg_scene_bindings* scene, const rtg_ray* rays, uint32_t count,
+                         uint32_t mode, uint32_t ray_flags, uint32_t cull_mask, uint32_t sbt_offset, rtg_hit* hits,
+                         uint32_t* occluded) {
+    qctx q;
+    if (qctx_init(&q, tlas, scene) != RTG_OK) return RTG_ERROR_INVALID_ARGUMENT;
+    q.ray_flags = ray_flags;
+    q.cull_mask = cull_mask & 0xffu;
+    q.sbt_offset = sbt_offset;
+    uint32_t ninst = q.th->prim_count;
+    for (uint32_t i = 0; i < count; ++i) {
+        const rtg_ray* r = &rays[i];
+        v3 o = V3(r->origin[0], r->origin[1], r->origin[2]);
+        v3 d = V3(r->direction[0], r->direction[1], r->direction[2]);
+        hitrec best;
+        best.t = r->tmax;
+        best.u = best.v = 0.0f;
+        best.inst = best.prim = best.geom = RTG_NO_HIT;
+        int found = 0;
+        for (uint32_t k = 0; k < ninst && !(found && mode == RTG_QUERY_OCCLUSION); ++k) {
+            const orc_inst* rec = &q.tinst[k];
+            if (((rec->mask_flags & 0xffu) & q.cull_mask) == 0 || !rec->blas) continue;
+            const orc_header* bh = (const orc_header*)(uintptr_t)rec->blas;
+            const orc_tri* tris = (const orc_tri*)((const uint8_t*)bh + bh->prims_off);
+            oray obj;
+            oray_setup(&obj, m34_point(rec->w2o, o), m34_dir(rec->w2o, d));
+            for (uint32_t t_i = 0; t_i < bh->prim_count; ++t_i) {
+                const orc_tri* tr = &tris[t_i];
+                float t, u, v;
+                if (!tri_test(&obj, tr->v0, tr->v1, tr->v2, r->tmin, best.t, &t, &u, &v)) continue;
+                uint32_t geom = tr->geom & ~ORC_NONOPAQUE_BIT;
+                if (mode == RTG_QUERY_CLOSEST &&
+                    !(t < best.t || (t == best.t && key_less(rec->index, geom, tr->prim, best.inst, best.geom, best.prim))))
+                    continue;
+                uint32_t iflags = (rec->mask_flags >> 8) & 0xffu;
+                if (!is_opaque(tr->geom, iflags, ray_flags)) {
+                    uint32_t group = (q.tsrc[rec->index].sbt_offset_and_flags & 0xffffffu) + sbt_offset;
+                    if (group < RTG_HIT_GROUP_COUNT && !alpha_clip_keep(&q, &q.cnt.anyhit_calls, rec->index, geom, tr->prim, u, v))
+                        continue;
+                }
+                found = 1;
+                if (mode == RTG_QUERY_OCCLUSION) break;
+                best.t = t;
+                best.u = u;
+                best.v = v;
+                best.inst = rec->index;
+                best.geom = geom;
+                best.prim = tr->prim;
+            }
+        }
+        if (mode == RTG_QUERY_OCCLUSION) {
+            occluded[i] = (uint32_t)found;
+        } else {
+            rtg_hit* h = &hits[i];
+            h->t = best.t;
+            h->instance = best.inst;
+            h->primitive = best.prim;
+            h->geometry = best.geom;
+            h->u = best.u;
+            h->v = best.v;
+        }
+    }
+    return RTG_OK;
+}
+
+/* ---------------------------------------------------------------- */
+/* Shading stages                                                    */
+/* ---------------------------------------------------------------- */
+
+/* pbr.glsl:56-65 and shaders/ray-tracing/src/pbr.rs:41-50 */
+float orc_v_smith_ggx_correlated(float NoV, float NoL, float roughness) {
+    float a2 = roughness * roughness;
+    float ggxv = NoL * sqrtf((NoV * NoV) * (1.0f - a2) + a2);
+    float ggxl = NoV * sqrtf((NoL * NoL) * (1.0f - a2) + a2);
+    return 0.5f / (ggxv + ggxl);
+}
+
+/* pbr.glsl:44-51 */
+float orc_d_ggx(float NoH, float roughness) {
+    float a = NoH * roughness;
+    float k = roughness / ((1.0f - NoH * NoH) + a * a);
+    return (k * k) * (1.0f / ORC_PI_F);
+}
+
+/* pbr.glsl:174-211 (brdf) with calculate_dot_params :30-39, F_Schlick
+ * :69-76, compute_f90/Fd_Burley :88-103. pow(x, 5.0) = ((x*x)*(x*x))*x. */
+static v3 brdf(v3 n, v3 view, v3 l, float pr, v3 base, float metallic, float reflectance, float intensity) {
+    v3 h = v3_normalize(v3_add(view, l));
+    float rough = pr * pr;
+    float NoV = clampf(v3_dot(n, view), 1e-9f, 1.0f);
+    float NoH = clampf(v3_dot(n, h), 0.0f, 1.0f);
+    float NoL = clampf(v3_dot(n, l), 0.0f, 1.0f);
+    float LoH = clampf(v3_dot(l, h), 0.0f, 1.0f);
+    float D = orc_d_ggx(NoH, rough);
+    float dielectric_f0 = (0.16f * reflectance) * reflectance;
+    v3 f0 = v3_add(V3(dielectric_f0 * (1.0f - metallic), dielectric_f0 * (1.0f - metallic),
+                      dielectric_f0 * (1.0f - metallic)),
+                   v3_scale(base, metallic));
+    float f90 = 0.5f + ((2.0f * rough) * LoH) * LoH;
+    float p5 = orc_pow5f(1.0f - LoH);
+    v3 F = V3(f0.x + (f90 - f0.x) * p5, f0.y + (f90 - f0.y) * p5, f0.z + (f90 - f0.z) * p5);
+    float Vis = orc_v_smith_ggx_correlated(NoV, NoL, rough);
+    float DV = D * Vis;
+    v3 spec = v3_scale(F, DV);
+    float ls = 1.0f + (f90 - 1.0f) * orc_pow5f(1.0f - NoL);
+    float vs = 1.0f + (f90 - 1.0f) * orc_pow5f(1.0f - NoV);
+    float fd = (ls * vs) * (1.0f / ORC_PI_F);
+    v3 diffuse = v3_scale(base, fd);
+    v3 comb = v3_add(diffuse, spec);
+    float in_nol = intensity * NoL;
+    return v3_scale(comb, in_nol);
+}
+
+void orc_brdf(const float normal[3], const float view[3], const float light[3], float perceptual_roughness,
+              const float base_colour[3], float metallic, float reflectance, float light_intensity, float out[3]) {
+    v3 r = brdf(V3(normal[0], normal[1], normal[2]), V3(view[0], view[1], view[2]), V3(light[0], light[1], light[2]),
+                perceptual_roughness, V3(base_colour[0], base_colour[1], base_colour[2]), metallic, reflectance,
+                light_intensity);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+
+/* shaders/ray-tracing/src/lib.rs:84-92 */
+static float srgb1(float x) {
+    if (x <= 0.0031308f) return x * 12.92f;
+    return 1.055f * orc_powf(x, 1.0f / 2.4f) - 0.055f;
+}
+void orc_linear_to_srgb(const float in[3], float out[3]) {
+    for (int i = 0; i < 3; ++i) out[i] = srgb1(in[i]);
+}
+
+static float smoothstep_(float e0, float e1, float x) {
+    float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return (t * t) * (3.0f - 2.0f * t);
+}
+
+/* shaders/ray-tracing/src/heatmap.rs:5-38 */
+void orc_heatmap_temperature(float heat, float out[3]) {
+    static const float col[10][3] = {
+        {0.0f, 2.0f, 91.0f}, {0.0f, 108.0f, 251.0f}, {0.0f, 221.0f, 221.0f}, {51.0f, 221.0f, 0.0f},
+        {255.0f, 252.0f, 0.0f}, {255.0f, 180.0f, 0.0f}, {255.0f, 104.0f, 0.0f}, {226.0f, 22.0f, 0.0f},
+        {191.0f, 0.0f, 83.0f}, {145.0f, 0.0f, 65.0f}};
+    float h = clampf(heat, 0.0f, 1.0f) * 10.0f;
+    int hi = (int)h;
+    int cur = hi, prv = hi - 1 > 0 ? hi - 1 : 0, nxt = hi + 1 < 9 ? hi + 1 : 9;
+    if (cur > 9) cur = 9; /* heat == 1.0 indexes one past the table in the reference */
+    float hf = floorf(h), hc = ceilf(h);
+    float blur = 0.8f;
+    float wc = smoothstep_(hf - blur, hf + blur, h) * (1.0f - smoothstep_(hc - blur, hc + blur, h));
+    float wp = 1.0f - smoothstep_(hf - blur, hf + blur, h);
+    float wn = smoothstep_(hc - blur, hc + blur, h);
+    for (int c = 0; c < 3; ++c) {
+        float r = (wc * (col[cur][c] / 255.0f) + wp * (col[prv][c] / 255.0f)) + wn * (col[nxt][c] / 255.0f);
+        out[c] = clampf(r, 0.0f, 1.0f);
+    }
+}
+
+/* closest_hit_textured.glsl:77-94 */
+static v3 sample_dir_light(float rx, float ry, v3 c, float radius) {
+    float rad = sqrtf(rx);
+    float angle = (ry * 2.0f) * ORC_PI_F;
+    float s, co;
+    orc_sincosf(angle, &s, &co);
+    float px = (rad * co) * radius, py = (rad * s) * radius;
+    v3 t = v3_normalize(v3_cross(c, V3(0.0f, 1.0f, 0.0f)));
+    v3 b = v3_normalize(v3_cross(t, c));
+    return v3_normalize(v3_add(v3_add(c, v3_scale(t, px)), v3_scale(b, py)));
+}
+void orc_sample_directional_light(const float rng[2], const float centre[3], float radius, float out[3]) {
+    v3 r = sample_dir_light(rng[0], rng[1], V3(centre[0], centre[1], centre[2]), radius);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+
+float orc_cos(float x) { return orc_cosf(x); }
+float orc_sin(float x) { return orc_sinf(x); }
+float orc_pow(float x, float y) { return orc_powf(x, y); }
+float orc_log2(float x) { return orc_log2f(x); }
+float orc_exp2(float y) { return orc_exp2f(y); }
+
+typedef struct {
+    const rtg_image_job* job;
+    const uint32_t* pixels;
+    uint32_t pixel_count;
+    float cos_sun;
+} image_ctx;
+
+static v3 load_v3(uint64_t base, uint32_t i) {
+    const float* p = (const float*)(uintptr_t)base + 3ull * i;
+    return V3(p[0], p[1], p[2]);
+}
+static v2 load_v2(uint64_t base, uint32_t i) {
+    const float* p = (const float*)(uintptr_t)base + 2ull * i;
+    v2 r = {p[0], p[1]};
+    return r;
+}
+
+/* closest_hit_textured main (closest_hit_textured.glsl:174-226). Returns the
+ * final colour given the hit; traces its own shadow rays. */
+static v3 shade_textured(qctx* q, const rtg_image_job* job, uint32_t px, uint32_t py, uint32_t frame,
+                         const hitrec* h, v3 ray_dir) {
+    const rtg_instance* src = &q->tsrc[h->inst];
+    uint32_t custom = src->custom_index_and_mask & 0xffffffu;
+    if (custom >= q->model_count) return V3(0.0f, 0.0f, 0.0f);
+    const rtg_model_info* mi = &q->models[custom];
+    const rtg_geometry_info* gi = (const rtg_geometry_info*)(uintptr_t)mi->geometry_info_address + h->geom;
+    const uint32_t* idx = (const uint32_t*)(uintptr_t)gi->index_buffer_address + 3ull * h->prim;
+    uint32_t i0 = idx[0], i1 = idx[1], i2 = idx[2];
+    v3 pa = load_v3(mi->position_buffer_address, i0), pb = load_v3(mi->position_buffer_address, i1),
+       pc = load_v3(mi->position_buffer_address, i2);
+    v3 na = load_v3(mi->normal_buffer_address, i0), nb = load_v3(mi->normal_buffer_address, i1),
+       nc = load_v3(mi->normal_buffer_address, i2);
+    v2 ua = load_v2(mi->uv_buffer_address, i0), ub = load_v2(mi->uv_buffer_address, i1),
+       uc = load_v2(mi->uv_buffer_address, i2);
+    v3 bc = V3((1.0f - h->u) - h->v, h->u, h->v);
+    v3 pos = v3_interp(pa, pb, pc, bc);
+    v3 nrm = v3_interp(na, nb, nc, bc);
+    v2 uv = v2_interp(ua, ub, uc, bc);
+
+    /* get_shadow_terminator_fix_shadow_origin (:13-39) */
+    v3 off[3];
+    v3 P[3] = {pa, pb, pc}, N[3] = {na, nb, nc};
+    for (int k = 0; k < 3; ++k) {
+        v3 vtp = v3_sub(pos, P[k]);
+        float dp = fminf(0.0f, v3_dot(vtp, N[k]));
+        off[k] = v3_sub(vtp, v3_scale(N[k], dp));
+    }
+    v3 fixed = v3_add(pos, v3_interp(off[0], off[1], off[2], bc));
+    const float* o2w = (const float*)src->transform;
+    v3 shadow_origin = m34_point(o2w, fixed);
+
+    const rtg_uniforms* U = &job->uniforms;
+    v3 sun = V3(U->sun_dir[0], U->sun_dir[1], U->sun_dir[2]);
+    uint32_t nsh = job->opts.shadow_rays;
+    float vis = 0.0f;
+    uint32_t saved_flags = q->ray_flags, saved_sbt = q->sbt_offset;
+    q->ray_flags = RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RTG_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER;
+    q->sbt_offset = 1;
+    for (uint32_t i = 0; i < nsh; ++i) {
+        /* sample_blue_noise (:99-111) + animated_blue_noise (:116-120) */
+        uint32_t o1x = i * 2u * 13u, o1y = i * 2u * 41u;
+        uint32_t o2x = (i * 2u + 1u) * 13u, o2y = (i * 2u + 1u) * 41u;
+        float t1[4], t2[4];
+        sample_index(q, U->blue_noise_texture_index, (float)(px + o1x) / 64.0f, (float)(py + o1y) / 64.0f, t1);
+        sample_index(q, U->blue_noise_texture_index, (float)(px + o2x) / 64.0f, (float)(py + o2y) / 64.0f, t2);
+        float anim = (float)(frame % 32u) * 0.618033988749f;
+        float rx = fractf(t1[0] + anim), ry = fractf(t2[0] + anim);
+        v3 sd = sample_dir_light(rx, ry, sun, U->sun_radius);
+        hitrec sh;
+        q->cnt.shadow_rays++;
+        int occl = traverse(q, shadow_origin, sd, 0.001f, 10000.0f, 1, &sh);
+        vis = vis + (occl > 0 ? 0.0f : 1.0f);
+    }
+    q->ray_flags = saved_flags;
+    q->sbt_offset = saved_sbt;
+    float sun_factor = vis / (float)nsh;
+
+    /* read_material_from_textures (:47-63) */
+    float dt[4], mr[4];
+    sample_index(q, gi->diffuse_image_index, uv.x, uv.y, dt);
+    sample_index(q, gi->metallic_roughness_image_index, uv.x, uv.y, mr);
+    v3 base = V3(dt[0], dt[1], dt[2]);
+    float metallic = mr[2], roughness = mr[1];
+
+    /* calculate_normal (:141-157) */
+    float w2o[12];
+    m34_inverse(o2w, w2o);
+    v3 normal;
+    if (gi->normal_map_image_index < 0) {
+        normal = v3_normalize(m34_transpose_dir(w2o, nrm));
+    } else {
+        float mt[4];
+        sample_index(q, (uint32_t)gi->normal_map_image_index, uv.x, uv.y, mt);
+        v3 mapn = V3(mt[0] * 2.0f - 1.0f, mt[1] * 2.0f - 1.0f, mt[2] * 2.0f - 1.0f);
+        /* compute_cotangent_frame (:123-139) */
+        v3 dp1 = v3_sub(pb, pa), dp2 = v3_sub(pc, pa);
+        float du1x = ub.x - ua.x, du1y = ub.y - ua.y, du2x = uc.x - ua.x, du2y = uc.y - ua.y;
+        v3 dp2perp = v3_cross(dp2, nrm), dp1perp = v3_cross(nrm, dp1);
+        v3 T = v3_add(v3_scale(dp2perp, du1x), v3_scale(dp1perp, du2x));
+        v3 B = v3_add(v3_scale(dp2perp, du1y), v3_scale(dp1perp, du2y));
+        float invmax = 1.0f / sqrtf(fmaxf(v3_dot(T, T), v3_dot(B, B)));
+        v3 tt = v3_scale(T, invmax), bb = v3_scale(B, invmax);
+        v3 m = v3_normalize(mapn);
+        v3 local = v3_add(v3_add(v3_scale(tt, m.x), v3_scale(bb, m.y)), v3_scale(nrm, m.z));
+        normal = v3_normalize(m34_transpose_dir(w2o, local));
+    }
+    v3 col = brdf(normal, v3_neg(ray_dir), sun, roughness, base, metallic, 0.5f, sun_factor);
+    v3 amb = V3(0.1f * base.x, 0.1f * base.y, 0.1f * base.z);
+    return v3_add(col, amb);
+}
+
+/* closest_hit_mirror (closest_hit_mirror.glsl:11-29): new direction. */
+static v3 mirror_dir(qctx* q, const hitrec* h, v3 d) {
+    const rtg_instance* src = &q->tsrc[h->inst];
+    uint32_t custom = src->custom_index_and_mask & 0xffffffu;
+    if (custom >= q->model_count) return V3(0.0f, 0.0f, 0.0f);
+    const rtg_model_info* mi = &q->models[custom];
+    const rtg_geometry_info* gi = (const rtg_geometry_info*)(uintptr_t)mi->geometry_info_address + h->geom;
+    const uint32_t* idx = (const uint32_t*)(uintptr_t)gi->index_buffer_address + 3ull * h->prim;
+    v3 na = load_v3(mi->normal_buffer_address, idx[0]), nb = load_v3(mi->normal_buffer_address, idx[1]),
+       nc = load_v3(mi->normal_buffer_address, idx[2]);
+    v3 bc = V3((1.0f - h->u) - h->v, h->u, h->v);
+    v3 n_i = v3_interp(na, nb, nc, bc);
+    float w2o[12];
+    m34_inverse((const float*)src->transform, w2o);
+    v3 n = v3_normalize(m34_transpose_dir(w2o, n_i));
+    float k = 2.0f * v3_dot(n, d);
+    return v3_sub(d, v3_scale(n, k));
+}
+
+/* One sample of one pixel: ray_generation (lib.rs:94-191). */
+static v3 trace_sample(qctx* q, const image_ctx* ic, uint32_t px, uint32_t py, uint32_t s, rtg_hit* primary) {
+    const rtg_image_job* job = ic->job;
+    const rtg_uniforms* U = &job->uniforms;
+    uint32_t spp = job->opts.spp;
+    float jx = 0.5f, jy = 0.5f;
+    if (spp > 1) {
+        jx = fractf(0.5f + (float)s * 0.754877669f);
+        jy = fractf(0.5f + (float)s * 0.569840296f);
+    }
+    uint32_t frame = U->frame_index * spp + s;
+    float pcx = (float)px + jx, pcy = (float)py + jy;
+    float tcx = pcx / (float)job->width, tcy = pcy / (float)job->height;
+    float rcx = tcx * 2.0f - 1.0f, rcy = tcy * 2.0f - 1.0f;
+    float zero_one[4] = {0.0f, 0.0f, 0.0f, 1.0f}, tmp[4];
+    mat4_mul_vec4(U->view_inverse, zero_one, tmp);
+    v3 origin = V3(tmp[0], tmp[1], tmp[2]);
+    float tv[4] = {rcx, rcy, 1.0f, 1.0f};
+    mat4_mul_vec4(U->proj_inverse, tv, tmp);
+    v3 ld = v3_normalize(V3(tmp[0], tmp[1], tmp[2]));
+    float dv[4] = {ld.x, ld.y, ld.z, 0.0f};
+    mat4_mul_vec4(U->view_inverse, dv, tmp);
+    v3 dir = V3(tmp[0], tmp[1], tmp[2]);
+    v3 colour = V3(0.0f, 0.0f, 0.0f);
+    for (uint32_t tr = 0; tr < job->opts.max_traces; ++tr) {
+        colour = V3(0.0f, 0.0f, 0.0f);
+        v3 no = V3(0.0f, 0.0f, 0.0f), nd = V3(0.0f, 0.0f, 0.0f);
+        hitrec h;
+        q->ray_flags = RTG_RAY_FLAG_NONE;
+        q->sbt_offset = 0;
+        if (tr == 0) q->cnt.primary_rays++;
+        else q->cnt.bounce_rays++;
+        int f = traverse(q, origin, dir, 0.01f, 10000.0f, 0, &h);
+        if (tr == 0 && primary) {
+            primary->t = h.t;
+            primary->instance = h.inst;
+            primary->primitive = h.prim;
+            primary->geometry = h.geom;
+            primary->u = h.u;
+            primary->v = h.v;
+        }
+        if (f <= 0) {
+            /* primary_ray_miss (lib.rs:40-51) */
+            v3 sun = V3(U->sun_dir[0], U->sun_dir[1], U->sun_dir[2]);
+            colour = v3_dot(dir, sun) > ic->cos_sun ? V3(1.0f, 1.0f, 1.0f) : V3(0.0f, 0.0f, 0.05f);
+        } else {
+            uint32_t group = q->tsrc[h.inst].sbt_offset_and_flags & 0xffffffu;
+            if (group == RTG_HIT_TEXTURED) {
+                colour = shade_textured(q, job, px, py, frame, &h, dir);
+            } else if (group == RTG_HIT_MIRROR) {
+                nd = mirror_dir(q, &h, dir);
+                no = v3_add(origin, v3_scale(dir, h.t));
+            } else if (group == RTG_HIT_PORTAL) {
+                /* closest_hit_portal (lib.rs:300-312) */
+                nd = dir;
+                no = v3_add(v3_add(origin, v3_scale(dir, h.t)), V3(0.0f, 5.0f, 0.0f));
+            }
+        }
+        if (nd.x == 0.0f && nd.y == 0.0f && nd.z == 0.0f) break;
+        origin = no;
+        dir = nd;
+    }
+    return colour;
+}
+
+#define TILE 16u
+
+static void image_item(void* a, qctx* q, uint32_t item) {
+    image_ctx* ic = (image_ctx*)a;
+    const rtg_image_job* job = ic->job;
+    uint32_t first, last;
+    if (ic->pixels) {
+        first = item * 64u;
+        last = first + 64u < ic->pixel_count ? first + 64u : ic->pixel_count;
+    } else {
+        first = 0;
+        last = 0;
+    }
+    uint32_t tiles_x = (job->width + TILE - 1) / TILE;
+    uint32_t n_work = ic->pixels ? last - first : TILE * TILE;
+    for (uint32_t w = 0; w < n_work; ++w) {
+        uint32_t px, py;
+        if (ic->pixels) {
+            uint32_t p = ic->pixels[first + w];
+            px = p % job->width;
+            py = p / job->width;
+        } else {
+            px = (item % tiles_x) * TILE + w % TILE;
+            py = (item / tiles_x) * TILE + w / TILE;
+            if (px >= job->width || py >= job->height) continue;
+        }
+        uint32_t pix = py * job->width + px;
+        v3 acc = V3(0.0f, 0.0f, 0.0f);
+        for (uint32_t s = 0; s < job->opts.spp; ++s) {
+            rtg_hit* ph = job->out_primary_hits ? &job->out_primary_hits[(uint64_t)pix * job->opts.spp + s] : NULL;
+            v3 c = trace_sample(q, ic, px, py, s, ph);
+            acc = v3_add(acc, c);
+        }
+        v3 col = job->opts.spp == 1 ? acc : v3_scale(acc, 1.0f / (float)job->opts.spp);
+        if (job->out_rgb) {
+            job->out_rgb[3ull * pix] = col.x;
+            job->out_rgb[3ull * pix + 1] = col.y;
+            job->out_rgb[3ull * pix + 2] = col.z;
+        }
+        if (job->out_rgba8) {
+            float lin[3] = {col.x, col.y, col.z}, sr[3];
+            orc_linear_to_srgb(lin, sr);
+            uint32_t r = (uint32_t)(clampf(sr[0], 0.0f, 1.0f) * 255.0f + 0.5f);
+            uint32_t g = (uint32_t)(clampf(sr[1], 0.0f, 1.0f) * 255.0f + 0.5f);
+            uint32_t b = (uint32_t)(clampf(sr[2], 0.0f, 1.0f) * 255.0f + 0.5f);
+            job->out_rgba8[pix] = r | (g << 8) | (b << 16) | (255u << 24);
+        }
+    }
+}
+
+int orc_trace_image(const rtg_image_job* job, const uint32_t* pixel_list, uint32_t pixel_count, int threads,
+                    rtg_trace_counters* counters) {
+    if (!job || job->width == 0 || job->height == 0 || job->opts.spp == 0 || job->opts.max_traces == 0 ||
+        job->opts.shadow_rays == 0)
+        return RTG_ERROR_INVALID_ARGUMENT;
+    image_ctx ic;
+    ic.job = job;
+    ic.pixels = pixel_list;
+    ic.pixel_count = pixel_count;
+    ic.cos_sun = orc_cosf(job->uniforms.sun_radius);
+    pool p;
+    memset(&p, 0, sizeof(p));
+    p.fn = image_item;
+    p.arg = &ic;
+    if (pixel_list) p.items = (pixel_count + 63u) / 64u;
+    else p.items = ((job->width + TILE - 1) / TILE) * ((job->height + TILE - 1) / TILE);
+    p.tlas = (const void*)(uintptr_t)job->tlas;
+    p.scene = &job->scene;
+    p.cull_mask = 0xffu;
+    run_pool(&p, threads);
+    if (counters) {
+        memset(counters, 0, sizeof(*counters));
+        add_counters(counters, &p.total);
+    }
+    return RTG_OK;
+}

@@ -1,0 +1,970 @@
+// build.hip -- acceleration-structure builds on gfx950.
+//
+// Replaces the driver work behind
+//   AccelerationStructure::build_blas   src/util_structs.rs:140-224
+//   build_tlas                          src/util_functions.rs:453-510
+//   AccelerationStructure::update_tlas  src/util_structs.rs:285-357 (refit)
+//   AccelerationStructure::clone        src/util_structs.rs:359-405
+// with a linear BVH (Karras 2012): primitive boxes and centres, 30-bit Morton
+// codes, a stable LSD radix sort written here (LDS digit ranking with wave
+// ballots), the radix-tree split search, and a bottom-up box merge with
+// atomic arrival counters. The node arrays are bit-identical to the host
+// oracle's: every box operation is an exact min/max or a fixed FP32 sequence.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "rtg_device.hpp"
+#include "rtg_internal.hpp"
+
+namespace rtg {
+
+// min/max of box planes as selects: the first operand wins ties, so that
+// +0/-0 resolve the same way on every device.
+__device__ inline float mins(float a, float b) { return b < a ? b : a; }
+__device__ inline float maxs(float a, float b) { return b > a ? b : a; }
+__device__ inline Box box_union(const Box& a, const Box& b) {
+    Box r;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        r.lo[k] = mins(a.lo[k], b.lo[k]);
+        r.hi[k] = maxs(a.hi[k], b.hi[k]);
+    }
+    return r;
+}
+__device__ inline Box box_empty() {
+    Box b;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        b.lo[k] = __builtin_inff();
+        b.hi[k] = -__builtin_inff();
+    }
+    return b;
+}
+
+// Order-preserving map of float to uint (for atomic min/max of centres).
+__device__ inline uint32_t f2ord(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float ord2f(uint32_t o) { return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o); }
+
+__device__ inline float centre(const Box& b, int a) { return (b.lo[a] + b.hi[a]) * 0.5f + 0.0f; }
+
+__device__ inline void set_child(Node& nd, int s, uint32_t ref, const Box& b) {
+    nd.x[2 * s] = b.lo[0];
+    nd.x[2 * s + 1] = b.hi[0];
+    nd.y[2 * s] = b.lo[1];
+    nd.y[2 * s + 1] = b.hi[1];
+    nd.z[2 * s] = b.lo[2];
+    nd.z[2 * s + 1] = b.hi[2];
+    nd.child[s] = ref;
+}
+
+__device__ inline Box node_box(const Node& nd) {
+    Box b;
+    b.lo[0] = mins(nd.x[0], nd.x[2]);
+    b.hi[0] = maxs(nd.x[1], nd.x[3]);
+    b.lo[1] = mins(nd.y[0], nd.y[2]);
+    b.hi[1] = maxs(nd.y[1], nd.y[3]);
+    b.lo[2] = mins(nd.z[0], nd.z[2]);
+    b.hi[2] = maxs(nd.z[1], nd.z[3]);
+    return b;
+}
+
+// Block-wide reduction of centre bounds, then one atomic per block and axis.
+template <int BS>
+__device__ inline void reduce_centre_bounds(float c[3], bool valid[3], uint32_t* cb) {
+    __shared__ uint32_t s_lo[3][BS / 64], s_hi[3][BS / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        uint32_t lo = valid[a] ? f2ord(c[a]) : 0xffffffffu;
+        uint32_t hi = valid[a] ? f2ord(c[a]) : 0u;
+        for (int off = 32; off > 0; off >>= 1) {
+            lo = min(lo, (uint32_t)__shfl_xor((int)lo, off));
+            hi = max(hi, (uint32_t)__shfl_xor((int)hi, off));
+        }
+        if (lane == 0) {
+            s_lo[a][w] = lo;
+            s_hi[a][w] = hi;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int a = threadIdx.x;
+        uint32_t lo = 0xffffffffu, hi = 0u;
+        for (int k = 0; k < BS / 64; ++k) {
+            lo = min(lo, s_lo[a][k]);
+            hi = max(hi, s_hi[a][k]);
+        }
+        if (lo != 0xffffffffu) atomicMin(&cb[a], lo);
+        if (hi != 0u) atomicMax(&cb[3 + a], hi);
+    }
+}
+
+struct GeomDesc {
+    const uint8_t* vertices;
+    const uint32_t* indices;
+    uint32_t stride;
+    uint32_t max_vertex;
+    uint32_t first_tri;
+    uint32_t flags;
+};
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_init_bounds(uint32_t* cb) {
+    if (threadIdx.x < 3) {
+        cb[threadIdx.x] = 0xffffffffu;
+        cb[3 + threadIdx.x] = 0u;
+    }
+}
+
+// Triangle records, primitive boxes, centre bounds.
+__global__ __launch_bounds__(256) void k_tri_setup(const GeomDesc* geoms, uint32_t ngeom, uint32_t ntri, TriRec* tris,
+                                                   Box* boxes, uint32_t* cb, uint32_t* status) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    float c[3] = {0, 0, 0};
+    bool valid[3] = {false, false, false};
+    if (i < ntri) {
+        uint32_t lo = 0, hi = ngeom;  // last geometry with first_tri <= i
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (geoms[mid].first_tri <= i) lo = mid;
+            else hi = mid;
+        }
+        const GeomDesc g = geoms[lo];
+        uint32_t t = i - g.first_tri;
+        uint32_t i0 = g.indices[3ull * t], i1 = g.indices[3ull * t + 1], i2 = g.indices[3ull * t + 2];
+        TriRec r;
+        Box b;
+        if (i0 > g.max_vertex || i1 > g.max_vertex || i2 > g.max_vertex) {
+            atomicOr(status, (uint32_t)kStatusBadIndex);
+            i0 = i1 = i2 = 0;
+        }
+        const float* a = reinterpret_cast<const float*>(g.vertices + (uint64_t)i0 * g.stride);
+        const float* bb = reinterpret_cast<const float*>(g.vertices + (uint64_t)i1 * g.stride);
+        const float* cc = reinterpret_cast<const float*>(g.vertices + (uint64_t)i2 * g.stride);
+        float A[3] = {a[0], a[1], a[2]}, B[3] = {bb[0], bb[1], bb[2]}, Cv[3] = {cc[0], cc[1], cc[2]};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            r.v0[k] = A[k];
+            r.v1[k] = B[k];
+            r.v2[k] = Cv[k];
+            b.lo[k] = mins(mins(A[k], B[k]), Cv[k]);
+            b.hi[k] = maxs(maxs(A[k], B[k]), Cv[k]);
+        }
+        r.prim = t;
+        r.geom = lo | ((g.flags & RTG_GEOMETRY_OPAQUE) ? 0u : kNonOpaqueBit);
+        r.pad = 0;
+        tris[i] = r;
+        boxes[i] = b;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            c[k] = centre(b, k);
+            valid[k] = isfinite(c[k]);
+        }
+    }
+    reduce_centre_bounds<256>(c, valid, cb);
+}
+
+__device__ inline bool registry_has(const uint64_t* reg, uint32_t n, uint64_t a) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (reg[mid] < a) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n && reg[lo] == a;
+}
+
+// World box of an instance: the 8 corners of its BLAS root box through the
+// object-to-world transform; empty for an empty or unknown BLAS.
+__device__ inline Box instance_world_box(const rtg_instance& in, const AsHeader* bh) {
+    Box r = box_empty();
+    if (bh == nullptr || bh->prim_count == 0) return r;
+    float m[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) m[k] = (&in.transform[0][0])[k];
+    for (int c = 0; c < 8; ++c) {
+        V3 p = v3((c & 1) ? bh->hi[0] : bh->lo[0], (c & 2) ? bh->hi[1] : bh->lo[1], (c & 4) ? bh->hi[2] : bh->lo[2]);
+        V3 w = m34_point(m, p);
+        r.lo[0] = mins(r.lo[0], w.x);
+        r.hi[0] = maxs(r.hi[0], w.x);
+        r.lo[1] = mins(r.lo[1], w.y);
+        r.hi[1] = maxs(r.hi[1], w.y);
+        r.lo[2] = mins(r.lo[2], w.z);
+        r.hi[2] = maxs(r.hi[2], w.z);
+    }
+    return r;
+}
+
+__device__ inline void fill_inst_rec(InstRec& rec, const rtg_instance& in, uint32_t index, uint64_t blas) {
+    float m[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) m[k] = (&in.transform[0][0])[k];
+    m34_inverse(m, rec.w2o);
+    rec.blas = blas;
+    rec.index = index;
+    rec.mask_flags = (in.custom_index_and_mask >> 24) | ((in.sbt_offset_and_flags >> 24) << 8);
+}
+
+__device__ inline const AsHeader* checked_blas(const rtg_instance& in, const uint64_t* reg, uint32_t nreg,
+                                               uint32_t* status) {
+    uint64_t a = in.acceleration_structure_reference;
+    if (a == 0) return nullptr;
+    if (!registry_has(reg, nreg, a)) {
+        atomicOr(status, (uint32_t)kStatusBadBlas);
+        return nullptr;
+    }
+    return reinterpret_cast<const AsHeader*>(a);
+}
+
+__global__ __launch_bounds__(256) void k_inst_setup(const rtg_instance* inst, uint32_t n, const uint64_t* reg,
+                                                    uint32_t nreg, InstRec* recs, Box* boxes, uint32_t* cb,
+                                                    uint32_t* status) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    float c[3] = {0, 0, 0};
+    bool valid[3] = {false, false, false};
+    if (i < n) {
+        rtg_instance in = inst[i];
+        const AsHeader* bh = checked_blas(in, reg, nreg, status);
+        Box b = instance_world_box(in, bh);
+        InstRec r;
+        fill_inst_rec(r, in, i, bh ? in.acceleration_structure_reference : 0);
+        recs[i] = r;
+        boxes[i] = b;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            c[k] = centre(b, k);
+            valid[k] = isfinite(c[k]);
+        }
+    }
+    reduce_centre_bounds<256>(c, valid, cb);
+}
+
+__device__ inline uint32_t spread3(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ inline uint32_t quantize10(float c, float lo, float hi) {
+    float ext = hi - lo;
+    float f = ext > 0.0f ? (c - lo) / ext : 0.0f;
+    float q = fminf(fmaxf(f * 1024.0f, 0.0f), 1023.0f);
+    return (uint32_t)q;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const Box* boxes, uint32_t n, const uint32_t* cb, uint32_t* keys,
+                                                uint32_t* vals) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = cb[a] == 0xffffffffu ? __builtin_inff() : ord2f(cb[a]);
+        hi[a] = cb[3 + a] == 0u ? -__builtin_inff() : ord2f(cb[3 + a]);
+    }
+    Box b = boxes[i];
+    uint32_t k = (spread3(quantize10(centre(b, 0), lo[0], hi[0])) << 2) |
+                 (spread3(quantize10(centre(b, 1), lo[1], hi[1])) << 1) | spread3(quantize10(centre(b, 2), lo[2], hi[2]));
+    keys[i] = k;
+    vals[i] = i;
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort, 8-bit digits, tiles of 256 threads x 16 keys.
+// ---------------------------------------------------------------------------
+constexpr int kRsThreads = 256;
+constexpr int kRsItems = 16;
+constexpr int kRsTile = kRsThreads * kRsItems;
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t* keys, uint32_t n, int shift, uint32_t* hist,
+                                                        uint32_t ntiles) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsItems; ++r) {
+        uint32_t i = base + r * kRsThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xffu], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of m entries in one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan_exclusive(uint32_t* data, uint32_t m) {
+    __shared__ uint32_t part[1024];
+    uint32_t chunk = (m + 1023) / 1024;
+    uint32_t b = threadIdx.x * chunk, e = min(m, b + chunk);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += data[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= (uint32_t)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s;
+    for (uint32_t i = b; i < e; ++i) {
+        uint32_t v = data[i];
+        data[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                                                           uint32_t* vout, uint32_t n, int shift,
+                                                           const uint32_t* offsets, uint32_t ntiles) {
+    constexpr int kWaves = kRsThreads / 64;
+    __shared__ uint32_t base_off[256];
+    __shared__ uint32_t running[256];
+    __shared__ uint32_t wave_cnt[kWaves][256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    base_off[threadIdx.x] = offsets[threadIdx.x * ntiles + blockIdx.x];
+    running[threadIdx.x] = 0;
+    for (int k = 0; k < kWaves; ++k) wave_cnt[k][threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint32_t tile = blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsItems; ++r) {
+        uint32_t i = tile + r * kRsThreads + threadIdx.x;
+        bool valid = i < n;
+        uint32_t key = valid ? kin[i] : 0u;
+        uint32_t val = valid ? vin[i] : 0u;
+        uint32_t digit = (key >> shift) & 0xffu;
+        // lanes of this wave holding the same digit
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            uint64_t bal = __ballot((digit >> bit) & 1u);
+            peers &= ((digit >> bit) & 1u) ? bal : ~bal;
+        }
+        uint32_t rank_in_wave = (uint32_t)__popcll(peers & lt_mask);
+        bool leader = valid && (peers & lt_mask) == 0;
+        if (leader) wave_cnt[w][digit] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = running[digit] + rank_in_wave;
+            for (int k = 0; k < w; ++k) off += wave_cnt[k][digit];
+            uint32_t dst = base_off[digit] + off;
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+        __syncthreads();
+        {
+            uint32_t d = threadIdx.x;
+            uint32_t s = 0;
+            for (int k = 0; k < kWaves; ++k) {
+                s += wave_cnt[k][d];
+                wave_cnt[k][d] = 0;
+            }
+            running[d] += s;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Radix tree (Karras 2012, Figure 4)
+// ---------------------------------------------------------------------------
+__device__ inline int delta(const uint32_t* keys, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = keys[i], b = keys[j];
+    if (a == b) return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+    return __clz((int)(a ^ b));
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint32_t* keys, int n, uint32_t* left, uint32_t* right,
+                                                uint32_t* parent) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(keys, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + (d < 0 ? d : 0);
+    int lo = min(i, j), hi = max(i, j);
+    uint32_t lc = (lo == gamma) ? (kLeafBit | (uint32_t)gamma) : (uint32_t)gamma;
+    uint32_t rc = (hi == gamma + 1) ? (kLeafBit | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+    left[i] = lc;
+    right[i] = rc;
+    // parent array: internal nodes at [0, n-1), leaves at [n-1, 2n-1)
+    parent[(lc & kLeafBit) ? (n - 1) + (lc & kFirstMask) : lc] = (uint32_t)i;
+    parent[(rc & kLeafBit) ? (n - 1) + (rc & kFirstMask) : rc] = (uint32_t)i;
+    if (i == 0) parent[0] = 0xffffffffu;
+}
+
+// Bottom-up boxes: the second child to arrive at a node finishes it.
+__global__ __launch_bounds__(256) void k_bottom_up(const Box* boxes, const uint32_t* order, int n, const uint32_t* left,
+                                                   const uint32_t* right, const uint32_t* parent, uint32_t* arrivals,
+                                                   uint32_t* height, Box* ibox, Node* nodes) {
+    int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    uint32_t node = parent[(n - 1) + k];
+    while (node != 0xffffffffu) {
+        __threadfence();
+        uint32_t old = atomicAdd(&arrivals[node], 1u);
+        if (old == 0) return;
+        __threadfence();
+        uint32_t c[2] = {left[node], right[node]};
+        Box cbx[2];
+        uint32_t hmax = 0;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            if (c[s] & kLeafBit) {
+                cbx[s] = boxes[order[c[s] & kFirstMask]];
+            } else {
+                cbx[s] = ibox[c[s]];
+                hmax = max(hmax, height[c[s]]);
+            }
+        }
+        Node nd;
+        set_child(nd, 0, c[0], cbx[0]);
+        set_child(nd, 1, c[1], cbx[1]);
+        nd.pad[0] = nd.pad[1] = 0;
+        nodes[node] = nd;
+        ibox[node] = box_union(cbx[0], cbx[1]);
+        height[node] = hmax + 1;
+        node = parent[node];
+    }
+}
+
+// Root node of a structure with at most one primitive.
+__global__ void k_small_root(const Box* boxes, uint32_t n, Node* nodes) {
+    if (threadIdx.x != 0) return;
+    Node nd;
+    Box e = box_empty();
+    set_child(nd, 0, n == 1 ? kLeafBit : kEmpty, n == 1 ? boxes[0] : e);
+    set_child(nd, 1, kEmpty, e);
+    nd.pad[0] = nd.pad[1] = 0;
+    nodes[0] = nd;
+}
+
+__global__ void k_header(AsHeader* h, uint32_t kind, uint32_t node_count, uint32_t prim_count, uint32_t geom_count,
+                         const uint32_t* height, uint32_t n, uint64_t nodes_off, uint64_t prims_off,
+                         uint64_t extra_off, uint64_t level_off, uint64_t leafbox_off, uint64_t total) {
+    if (threadIdx.x != 0) return;
+    AsHeader hd;
+    memset(&hd, 0, sizeof(hd));
+    hd.magic = kMagic;
+    hd.kind = kind;
+    hd.node_count = node_count;
+    hd.prim_count = prim_count;
+    hd.geom_count = geom_count;
+    hd.max_depth = n >= 2 ? height[0] : 1u;
+    hd.node_width = 2;
+    hd.flags = 0;
+    const Node* nodes = reinterpret_cast<const Node*>(reinterpret_cast<const uint8_t*>(h) + nodes_off);
+    Box r = node_box(nodes[0]);
+    for (int a = 0; a < 3; ++a) {
+        hd.lo[a] = r.lo[a];
+        hd.hi[a] = r.hi[a];
+    }
+    hd.nodes_off = nodes_off;
+    hd.prims_off = prims_off;
+    hd.extra_off = extra_off;
+    hd.level_off = level_off;
+    hd.total_size = total;
+    hd.leafbox_off = leafbox_off;
+    *h = hd;
+}
+
+__global__ __launch_bounds__(256) void k_gather_tris(const uint32_t* order, const TriRec* in, TriRec* out, uint32_t n) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = in[order[i]];
+}
+
+__global__ __launch_bounds__(256) void k_gather_insts(const uint32_t* order, const InstRec* in, const Box* boxes,
+                                                      InstRec* out, Box* leafbox, uint32_t n) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        uint32_t s = order[i];
+        out[i] = in[s];
+        leafbox[i] = boxes[s];
+    }
+}
+
+// Level lists of the internal nodes, deepest level first, ascending index
+// within a level (one block; used once per TLAS build).
+__global__ __launch_bounds__(1024) void k_tlas_levels(const uint32_t* parent, uint32_t nint, uint32_t* depth,
+                                                      uint32_t* level_data) {
+    __shared__ uint32_t s_max;
+    if (threadIdx.x == 0) s_max = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nint; i += blockDim.x) {
+        uint32_t dpt = 0;
+        for (uint32_t p = parent[i]; p != 0xffffffffu; p = parent[p]) ++dpt;
+        depth[i] = dpt;
+        atomicMax(&s_max, dpt);
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    uint32_t levels = nint ? s_max + 1 : 0;
+    level_data[0] = levels;
+    uint32_t* offs = level_data + 1;
+    uint32_t* list = level_data + 2 + levels;
+    // counts per level (deepest level first)
+    for (uint32_t L = 0; L <= levels; ++L) offs[L] = 0;
+    for (uint32_t i = 0; i < nint; ++i) offs[levels - 1 - depth[i] + 1]++;
+    for (uint32_t L = 0; L < levels; ++L) offs[L + 1] += offs[L];
+    // fill in ascending node order; offs[L] is used as a cursor then restored
+    for (uint32_t i = 0; i < nint; ++i) {
+        uint32_t L = levels - 1 - depth[i];
+        list[offs[L]++] = i;
+    }
+    for (uint32_t L = levels; L > 0; --L) offs[L] = offs[L - 1];
+    offs[0] = 0;
+}
+
+// In-place TLAS UPDATE (src/util_structs.rs:285-357): new instance records,
+// same topology. One workgroup walks the level lists from the deepest level
+// to the root with a barrier between levels.
+__global__ __launch_bounds__(1024) void k_tlas_refit(uint8_t* base, const rtg_instance* inst, uint32_t n,
+                                                     const uint64_t* reg, uint32_t nreg, uint32_t* status) {
+    AsHeader* h = reinterpret_cast<AsHeader*>(base);
+    Node* nodes = reinterpret_cast<Node*>(base + h->nodes_off);
+    InstRec* recs = reinterpret_cast<InstRec*>(base + h->prims_off);
+    rtg_instance* src = reinterpret_cast<rtg_instance*>(base + h->extra_off);
+    Box* leafbox = reinterpret_cast<Box*>(base + h->leafbox_off);
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        uint32_t s = recs[k].index;
+        rtg_instance in = inst[s];
+        const AsHeader* bh = checked_blas(in, reg, nreg, status);
+        InstRec r;
+        fill_inst_rec(r, in, s, bh ? in.acceleration_structure_reference : 0);
+        recs[k] = r;
+        leafbox[k] = instance_world_box(in, bh);
+        src[s] = in;
+    }
+    __syncthreads();
+    if (n >= 2) {
+        const uint32_t* lv = reinterpret_cast<const uint32_t*>(base + h->level_off);
+        uint32_t levels = lv[0];
+        const uint32_t* offs = lv + 1;
+        const uint32_t* list = lv + 2 + levels;
+        for (uint32_t L = 0; L < levels; ++L) {
+            for (uint32_t q = offs[L] + threadIdx.x; q < offs[L + 1]; q += blockDim.x) {
+                Node& nd = nodes[list[q]];
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    uint32_t c = nd.child[s];
+                    Box cb = (c & kLeafBit) ? leafbox[c & kFirstMask] : node_box(nodes[c]);
+                    set_child(nd, s, c, cb);
+                }
+            }
+            __syncthreads();
+        }
+    } else if (n == 1 && threadIdx.x == 0) {
+        set_child(nodes[0], 0, kLeafBit, leafbox[0]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Box r = node_box(nodes[0]);
+        for (int a = 0; a < 3; ++a) {
+            h->lo[a] = r.lo[a];
+            h->hi[a] = r.hi[a];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static inline uint64_t align64(uint64_t x) { return (x + 63) & ~uint64_t(63); }
+static inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+struct Layout {
+    uint64_t nodes_off, prims_off, extra_off, level_off, leafbox_off, total;
+    uint32_t node_count;
+};
+
+static Layout blas_layout(uint32_t n, uint32_t ngeom) {
+    Layout L;
+    L.node_count = n >= 2 ? n - 1 : 1;
+    L.nodes_off = 128;
+    L.prims_off = align64(L.nodes_off + (uint64_t)L.node_count * 64);
+    L.extra_off = align64(L.prims_off + (uint64_t)n * 48);
+    L.level_off = 0;
+    L.leafbox_off = 0;
+    L.total = align64(L.extra_off + (uint64_t)ngeom * 4);
+    return L;
+}
+
+static Layout tlas_layout(uint32_t n) {
+    Layout L;
+    uint32_t nint = n >= 2 ? n - 1 : 0;
+    L.node_count = n >= 2 ? n - 1 : 1;
+    L.nodes_off = 128;
+    L.prims_off = align64(L.nodes_off + (uint64_t)L.node_count * 64);
+    L.extra_off = align64(L.prims_off + (uint64_t)n * 64);
+    L.level_off = align64(L.extra_off + (uint64_t)n * 64);
+    L.leafbox_off = align64(L.level_off + 4ull * (1 + (nint + 1) + nint));
+    L.total = align64(L.leafbox_off + (uint64_t)n * 24);
+    return L;
+}
+
+// Scratch needs of the generic LBVH over n primitives.
+static size_t lbvh_scratch_bytes(uint32_t n, size_t rec_bytes) {
+    size_t off = 0;
+    uint32_t ntiles = (n + kRsTile - 1) / kRsTile;
+    carve_size<uint8_t>(off, (size_t)n * rec_bytes);  // unsorted records
+    carve_size<Box>(off, n);                          // primitive boxes
+    carve_size<uint32_t>(off, 8);                     // centre bounds
+    carve_size<uint32_t>(off, n);                     // keys a
+    carve_size<uint32_t>(off, n);                     // keys b
+    carve_size<uint32_t>(off, n);                     // vals a
+    carve_size<uint32_t>(off, n);                     // vals b
+    carve_size<uint32_t>(off, (size_t)256 * std::max(1u, ntiles));
+    carve_size<uint32_t>(off, n);                     // left
+    carve_size<uint32_t>(off, n);                     // right
+    carve_size<uint32_t>(off, 2ull * n);              // parent
+    carve_size<uint32_t>(off, n);                     // arrivals
+    carve_size<uint32_t>(off, n);                     // height
+    carve_size<Box>(off, n);                          // internal boxes
+    carve_size<uint32_t>(off, n);                     // depth
+    carve_size<GeomDesc>(off, 1);
+    return off + 4096;
+}
+
+struct LbvhBufs {
+    uint8_t* recs;
+    Box* boxes;
+    uint32_t* cb;
+    uint32_t *ka, *kb, *va, *vb, *hist, *left, *right, *parent, *arrivals, *height, *depth;
+    Box* ibox;
+    GeomDesc* geoms;
+};
+
+static LbvhBufs carve_lbvh(void* base, uint32_t n, size_t rec_bytes, uint32_t ngeom = 1) {
+    Carve c(base);
+    LbvhBufs b;
+    b.geoms = c.take<GeomDesc>(std::max(1u, ngeom));
+    uint32_t ntiles = (n + kRsTile - 1) / kRsTile;
+    b.recs = c.take<uint8_t>((size_t)n * rec_bytes);
+    b.boxes = c.take<Box>(n);
+    b.cb = c.take<uint32_t>(8);
+    b.ka = c.take<uint32_t>(n);
+    b.kb = c.take<uint32_t>(n);
+    b.va = c.take<uint32_t>(n);
+    b.vb = c.take<uint32_t>(n);
+    b.hist = c.take<uint32_t>((size_t)256 * std::max(1u, ntiles));
+    b.left = c.take<uint32_t>(n);
+    b.right = c.take<uint32_t>(n);
+    b.parent = c.take<uint32_t>(2ull * n);
+    b.arrivals = c.take<uint32_t>(n);
+    b.height = c.take<uint32_t>(n);
+    b.ibox = c.take<Box>(n);
+    b.depth = c.take<uint32_t>(n);
+    return b;
+}
+
+// Sort keys/values, build the radix tree and its boxes into `nodes`.
+// On return the sorted primitive order is in b.va.
+static int lbvh_hierarchy(LbvhBufs& b, uint32_t n, Node* nodes, hipStream_t s) {
+    if (n <= 1) {
+        hipLaunchKernelGGL(k_small_root, dim3(1), dim3(64), 0, s, b.boxes, n, nodes);
+        if (n == 1) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, b.va, 1ull, 0u);
+        RTG_HIP(hipGetLastError());
+        return RTG_OK;
+    }
+    hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.boxes, n, b.cb, b.ka, b.va);
+    uint32_t ntiles = (n + kRsTile - 1) / kRsTile;
+    uint32_t *kin = b.ka, *kout = b.kb, *vin = b.va, *vout = b.vb;
+    for (int pass = 0; pass < 4; ++pass) {
+        int shift = pass * 8;
+        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(kRsThreads), 0, s, kin, n, shift, b.hist, ntiles);
+        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, b.hist, 256u * ntiles);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(kRsThreads), 0, s, kin, vin, kout, vout, n, shift, b.hist,
+                           ntiles);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    // after 4 passes the sorted data is back in (ka, va)
+    hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, b.ka, (int)n, b.left, b.right,
+                       b.parent);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.arrivals, (uint64_t)n, 0u);
+    hipLaunchKernelGGL(k_bottom_up, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.boxes, b.va, (int)n, b.left, b.right,
+                       b.parent, b.arrivals, b.height, b.ibox, nodes);
+    RTG_HIP(hipGetLastError());
+    return RTG_OK;
+}
+
+int refresh_registry(rtg_ctx* ctx) {
+    std::vector<uint64_t> addrs;
+    for (auto& kv : ctx->structures)
+        if (kv.second.kind == kKindBlas) addrs.push_back(kv.first);
+    std::sort(addrs.begin(), addrs.end());
+    uint64_t* d = nullptr;
+    RTG_HIP(hipMalloc(&d, std::max<size_t>(1, addrs.size()) * sizeof(uint64_t)));
+    if (!addrs.empty()) RTG_HIP(hipMemcpy(d, addrs.data(), addrs.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    if (ctx->d_registry) ctx->retired.push_back(ctx->d_registry);
+    ctx->d_registry = d;
+    ctx->registry_count = (uint32_t)addrs.size();
+    return RTG_OK;
+}
+
+int check_status(rtg_ctx* ctx, hipStream_t stream, uint32_t mask, const char* what) {
+    uint32_t st = 0;
+    RTG_HIP(hipStreamSynchronize(stream));
+    RTG_HIP(hipMemcpy(&st, ctx->d_status, 4, hipMemcpyDeviceToHost));
+    if (st & mask) {
+        uint32_t cleared = st & ~mask;
+        RTG_HIP(hipMemcpy(ctx->d_status, &cleared, 4, hipMemcpyHostToDevice));
+        set_error("%s: device reported status 0x%x (1: index > max_vertex, 2: reference to an unknown BLAS, "
+                  "4: traversal stack overflow)",
+                  what, st & mask);
+        return RTG_ERROR_INVALID_ARGUMENT;
+    }
+    return RTG_OK;
+}
+
+}  // namespace rtg
+
+using namespace rtg;
+
+extern "C" int rtg_blas_query_sizes(rtg_ctx* ctx, const rtg_geometry* geoms, uint32_t n_geom, rtg_build_sizes* out) {
+    clear_error();
+    RTG_REQUIRE(ctx && out && (geoms || n_geom == 0), RTG_ERROR_INVALID_ARGUMENT, "rtg_blas_query_sizes: null argument");
+    uint64_t total = 0;
+    for (uint32_t g = 0; g < n_geom; ++g) total += geoms[g].index_count / 3;
+    RTG_REQUIRE(total < (1ull << 27), RTG_ERROR_LIMIT, "BLAS has %llu triangles (limit 2^27)",
+                (unsigned long long)total);
+    Layout L = blas_layout((uint32_t)total, n_geom);
+    out->acceleration_structure_size = L.total;
+    out->build_scratch_size = lbvh_scratch_bytes((uint32_t)total, sizeof(TriRec)) + n_geom * sizeof(GeomDesc);
+    out->update_scratch_size = 0;
+    return RTG_OK;
+}
+
+extern "C" int rtg_tlas_query_sizes(rtg_ctx* ctx, uint32_t count, rtg_build_sizes* out) {
+    clear_error();
+    RTG_REQUIRE(ctx && out, RTG_ERROR_INVALID_ARGUMENT, "rtg_tlas_query_sizes: null argument");
+    RTG_REQUIRE(count < (1u << 27), RTG_ERROR_LIMIT, "TLAS has %u instances (limit 2^27)", count);
+    Layout L = tlas_layout(count);
+    out->acceleration_structure_size = L.total;
+    out->build_scratch_size = lbvh_scratch_bytes(count, sizeof(InstRec));
+    out->update_scratch_size = 0;
+    return RTG_OK;
+}
+
+extern "C" int rtg_blas_build(rtg_ctx* ctx, const rtg_geometry* geoms, uint32_t n_geom, uint32_t flags, rtg_blas* out,
+                              void* stream) {
+    clear_error();
+    RTG_REQUIRE(ctx && out && (geoms || n_geom == 0), RTG_ERROR_INVALID_ARGUMENT, "rtg_blas_build: null argument");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    RTG_HIP(hipSetDevice(ctx->device));
+    uint64_t total = 0;
+    std::vector<GeomDesc> gd(std::max(1u, n_geom));
+    for (uint32_t g = 0; g < n_geom; ++g) {
+        const rtg_geometry& G = geoms[g];
+        RTG_REQUIRE(G.index_count % 3 == 0, RTG_ERROR_INVALID_ARGUMENT,
+                    "geometry %u: index_count %u is not a multiple of 3", g, G.index_count);
+        RTG_REQUIRE(G.vertex_stride >= 12 && G.vertex_stride % 4 == 0, RTG_ERROR_INVALID_ARGUMENT,
+                    "geometry %u: vertex_stride %u (need >= 12, multiple of 4)", g, G.vertex_stride);
+        RTG_REQUIRE(G.index_count == 0 || (G.vertex_data && G.index_data), RTG_ERROR_INVALID_ARGUMENT,
+                    "geometry %u: null vertex or index data", g);
+        gd[g].vertices = reinterpret_cast<const uint8_t*>(G.vertex_data);
+        gd[g].indices = reinterpret_cast<const uint32_t*>(G.index_data);
+        gd[g].stride = G.vertex_stride;
+        gd[g].max_vertex = G.max_vertex;
+        gd[g].first_tri = (uint32_t)total;
+        gd[g].flags = G.flags;
+        total += G.index_count / 3;
+        RTG_REQUIRE(total < (1ull << 27), RTG_ERROR_LIMIT, "BLAS exceeds 2^27 triangles");
+    }
+    uint32_t n = (uint32_t)total;
+    Layout L = blas_layout(n, n_geom);
+    uint8_t* blob = nullptr;
+    RTG_HIP(hipMalloc(&blob, L.total));
+    RTG_HIP(hipMemsetAsync(blob, 0, L.total, s));
+    size_t need = lbvh_scratch_bytes(n, sizeof(TriRec)) + std::max(1u, n_geom) * sizeof(GeomDesc) + 256;
+    if (ctx->scratch.ensure(need) != RTG_OK) {
+        hipFree(blob);
+        return RTG_ERROR_OUT_OF_MEMORY;
+    }
+    LbvhBufs b = carve_lbvh(ctx->scratch.ptr, n, sizeof(TriRec), n_geom);
+    GeomDesc* d_geoms = b.geoms;
+    // the copy source must outlive the asynchronous copy: the build
+    // synchronises its stream before returning
+    RTG_HIP(hipMemcpyAsync(d_geoms, gd.data(), std::max(1u, n_geom) * sizeof(GeomDesc), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, b.cb);
+    if (n > 0)
+        hipLaunchKernelGGL(k_tri_setup, dim3(blocks_for(n, 256)), dim3(256), 0, s, d_geoms, n_geom, n,
+                           reinterpret_cast<TriRec*>(b.recs), b.boxes, b.cb, ctx->d_status);
+    Node* nodes = reinterpret_cast<Node*>(blob + L.nodes_off);
+    int st = lbvh_hierarchy(b, n, nodes, s);
+    if (st != RTG_OK) {
+        hipFree(blob);
+        return st;
+    }
+    if (n > 0)
+        hipLaunchKernelGGL(k_gather_tris, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.va,
+                           reinterpret_cast<const TriRec*>(b.recs), reinterpret_cast<TriRec*>(blob + L.prims_off), n);
+    std::vector<uint32_t> gflags(std::max(1u, n_geom));
+    for (uint32_t g = 0; g < n_geom; ++g) gflags[g] = geoms[g].flags;
+    if (n_geom) RTG_HIP(hipMemcpyAsync(blob + L.extra_off, gflags.data(), n_geom * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_header, dim3(1), dim3(64), 0, s, reinterpret_cast<AsHeader*>(blob), kKindBlas, L.node_count,
+                       n, n_geom, b.height, n, L.nodes_off, L.prims_off, L.extra_off, L.level_off, L.leafbox_off,
+                       L.total);
+    RTG_HIP(hipGetLastError());
+    st = check_status(ctx, s, kStatusBadIndex, "rtg_blas_build");
+    if (st != RTG_OK) {
+        hipFree(blob);
+        return st;
+    }
+    uint64_t addr = reinterpret_cast<uint64_t>(blob);
+    ctx->structures[addr] = AsRecord{kKindBlas, n, L.node_count, L.total};
+    st = refresh_registry(ctx);
+    if (st != RTG_OK) return st;
+    (void)flags;
+    *out = addr;
+    return RTG_OK;
+}
+
+extern "C" int rtg_tlas_build(rtg_ctx* ctx, const rtg_instance* instances, uint32_t count, uint32_t flags,
+                              rtg_tlas* out, void* stream) {
+    clear_error();
+    RTG_REQUIRE(ctx && out && (instances || count == 0), RTG_ERROR_INVALID_ARGUMENT, "rtg_tlas_build: null argument");
+    RTG_REQUIRE(count < (1u << 27), RTG_ERROR_LIMIT, "TLAS has %u instances (limit 2^27)", count);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    RTG_HIP(hipSetDevice(ctx->device));
+    uint32_t n = count;
+    Layout L = tlas_layout(n);
+    uint8_t* blob = nullptr;
+    RTG_HIP(hipMalloc(&blob, L.total));
+    RTG_HIP(hipMemsetAsync(blob, 0, L.total, s));
+    size_t need = lbvh_scratch_bytes(n, sizeof(InstRec));
+    if (ctx->scratch.ensure(need) != RTG_OK) {
+        hipFree(blob);
+        return RTG_ERROR_OUT_OF_MEMORY;
+    }
+    LbvhBufs b = carve_lbvh(ctx->scratch.ptr, n, sizeof(InstRec));
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, b.cb);
+    if (n > 0)
+        hipLaunchKernelGGL(k_inst_setup, dim3(blocks_for(n, 256)), dim3(256), 0, s, instances, n, ctx->d_registry,
+                           ctx->registry_count, reinterpret_cast<InstRec*>(b.recs), b.boxes, b.cb, ctx->d_status);
+    Node* nodes = reinterpret_cast<Node*>(blob + L.nodes_off);
+    int st = lbvh_hierarchy(b, n, nodes, s);
+    if (st != RTG_OK) {
+        hipFree(blob);
+        return st;
+    }
+    if (n > 0) {
+        hipLaunchKernelGGL(k_gather_insts, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.va,
+                           reinterpret_cast<const InstRec*>(b.recs), b.boxes,
+                           reinterpret_cast<InstRec*>(blob + L.prims_off), reinterpret_cast<Box*>(blob + L.leafbox_off),
+                           n);
+        RTG_HIP(hipMemcpyAsync(blob + L.extra_off, instances, (size_t)n * 64, hipMemcpyDeviceToDevice, s));
+    }
+    uint32_t nint = n >= 2 ? n - 1 : 0;
+    hipLaunchKernelGGL(k_tlas_levels, dim3(1), dim3(1024), 0, s, b.parent, nint, b.depth,
+                       reinterpret_cast<uint32_t*>(blob + L.level_off));
+    hipLaunchKernelGGL(k_header, dim3(1), dim3(64), 0, s, reinterpret_cast<AsHeader*>(blob), kKindTlas, L.node_count,
+                       n, 0u, b.height, n, L.nodes_off, L.prims_off, L.extra_off, L.level_off, L.leafbox_off, L.total);
+    RTG_HIP(hipGetLastError());
+    st = check_status(ctx, s, kStatusBadBlas, "rtg_tlas_build");
+    if (st != RTG_OK) {
+        hipFree(blob);
+        return st;
+    }
+    uint64_t addr = reinterpret_cast<uint64_t>(blob);
+    ctx->structures[addr] = AsRecord{kKindTlas, n, L.node_count, L.total};
+    (void)flags;
+    *out = addr;
+    return RTG_OK;
+}
+
+extern "C" int rtg_tlas_refit(rtg_ctx* ctx, rtg_tlas tlas, const rtg_instance* instances, uint32_t count,
+                              void* stream) {
+    clear_error();
+    RTG_REQUIRE(ctx && (instances || count == 0), RTG_ERROR_INVALID_ARGUMENT, "rtg_tlas_refit: null argument");
+    auto it = ctx->structures.find(tlas);
+    RTG_REQUIRE(it != ctx->structures.end() && it->second.kind == kKindTlas, RTG_ERROR_NOT_FOUND,
+                "rtg_tlas_refit: 0x%llx is not a TLAS of this context", (unsigned long long)tlas);
+    RTG_REQUIRE(it->second.prim_count == count, RTG_ERROR_INVALID_ARGUMENT,
+                "rtg_tlas_refit: %u instances, the TLAS was built with %u", count, it->second.prim_count);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    LaunchTimer timer(ctx, RTG_KERNEL_REFIT, s);
+    hipLaunchKernelGGL(k_tlas_refit, dim3(1), dim3(1024), 0, s, reinterpret_cast<uint8_t*>(tlas), instances, count,
+                       ctx->d_registry, ctx->registry_count, ctx->d_status);
+    RTG_HIP(hipGetLastError());
+    return RTG_OK;
+}
+
+extern "C" int rtg_tlas_clone(rtg_ctx* ctx, rtg_tlas src, rtg_tlas* out, void* stream) {
+    clear_error();
+    RTG_REQUIRE(ctx && out, RTG_ERROR_INVALID_ARGUMENT, "rtg_tlas_clone: null argument");
+    auto it = ctx->structures.find(src);
+    RTG_REQUIRE(it != ctx->structures.end() && it->second.kind == kKindTlas, RTG_ERROR_NOT_FOUND,
+                "rtg_tlas_clone: 0x%llx is not a TLAS of this context", (unsigned long long)src);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    RTG_HIP(hipSetDevice(ctx->device));
+    uint8_t* blob = nullptr;
+    RTG_HIP(hipMalloc(&blob, it->second.size));
+    RTG_HIP(hipMemcpyAsync(blob, reinterpret_cast<const void*>(src), it->second.size, hipMemcpyDeviceToDevice, s));
+    uint64_t addr = reinterpret_cast<uint64_t>(blob);
+    ctx->structures[addr] = it->second;
+    *out = addr;
+    return RTG_OK;
+}
+
+static int destroy_as(rtg_ctx* ctx, uint64_t addr, uint32_t kind, const char* what) {
+    clear_error();
+    RTG_REQUIRE(ctx, RTG_ERROR_INVALID_ARGUMENT, "%s: null context", what);
+    auto it = ctx->structures.find(addr);
+    RTG_REQUIRE(it != ctx->structures.end() && it->second.kind == kind, RTG_ERROR_NOT_FOUND,
+                "%s: 0x%llx is not a live structure of this kind", what, (unsigned long long)addr);
+    RTG_HIP(hipDeviceSynchronize());
+    RTG_HIP(hipFree(reinterpret_cast<void*>(addr)));
+    ctx->structures.erase(it);
+    if (kind == kKindBlas) return refresh_registry(ctx);
+    return RTG_OK;
+}
+
+extern "C" int rtg_blas_destroy(rtg_ctx* ctx, rtg_blas blas) { return destroy_as(ctx, blas, kKindBlas, "rtg_blas_destroy"); }
+extern "C" int rtg_tlas_destroy(rtg_ctx* ctx, rtg_tlas tlas) { return destroy_as(ctx, tlas, kKindTlas, "rtg_tlas_destroy"); }
+
+extern "C" int rtg_as_get_info(rtg_ctx* ctx, uint64_t addr, rtg_as_info* out) {
+    clear_error();
+    RTG_REQUIRE(ctx && out, RTG_ERROR_INVALID_ARGUMENT, "rtg_as_get_info: null argument");
+    auto it = ctx->structures.find(addr);
+    RTG_REQUIRE(it != ctx->structures.end(), RTG_ERROR_NOT_FOUND, "rtg_as_get_info: unknown structure 0x%llx",
+                (unsigned long long)addr);
+    AsHeader h;
+    RTG_HIP(hipDeviceSynchronize());
+    RTG_HIP(hipMemcpy(&h, reinterpret_cast<const void*>(addr), sizeof(h), hipMemcpyDeviceToHost));
+    out->kind = h.kind;
+    out->node_count = h.node_count;
+    out->prim_count = h.prim_count;
+    out->geometry_count = h.geom_count;
+    out->max_depth = h.max_depth;
+    out->node_width = h.node_width;
+    for (int a = 0; a < 3; ++a) {
+        out->bounds[a] = h.lo[a];
+        out->bounds[3 + a] = h.hi[a];
+    }
+    out->size_bytes = h.total_size;
+    return RTG_OK;
+}

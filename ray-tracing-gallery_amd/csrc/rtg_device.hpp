@@ -1,0 +1,616 @@
+// rtg_device.hpp -- device code shared by the build and trace kernels (gfx950).
+//
+// Everything here is compiled with -ffp-contract=off and correctly rounded
+// FP32 division and square root, so each written operation is one IEEE-754
+// binary32 operation rounded to nearest even. The operation order of every
+// function is part of the contract: the CPU oracle evaluates the same
+// operations, and hit IDs and occlusion must agree bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rtg.h"
+
+namespace rtg {
+
+// ---------------------------------------------------------------------------
+// Acceleration-structure blob layout (one device allocation per structure).
+//   header (128 B) | nodes (64 B) | prims | extra [| level data | leaf boxes]
+// BLAS prims: 48-B triangle records in leaf order
+//   {v0.xyz, primitive id} {v1.xyz, geometry id | non-opaque bit 31} {v2.xyz, 0}
+// TLAS prims: 64-B instance leaf records in leaf order
+//   {world-to-object 3x4 row-major} {BLAS address, instance index, mask | flags<<8}
+// TLAS extra: copy of the source rtg_instance records (original order).
+// TLAS level data: level_count, level_offsets[level_count+1], level_nodes[]
+//   (internal nodes, deepest level first, ascending index within a level).
+// TLAS leaf boxes: world box of each leaf (24 B), scratch of the refit.
+// Node: x planes (lo0, hi0, lo1, hi1), y planes, z planes, child refs.
+// Child ref: internal node index, or LEAF_BIT | (count-1) << 27 | first,
+//   or EMPTY.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kMagic = 0x53415452u;  // 'RTAS'
+constexpr uint32_t kKindBlas = 0u;
+constexpr uint32_t kKindTlas = 1u;
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kSentinel = 0xFFFFFFFEu;
+constexpr uint32_t kFirstMask = 0x07FFFFFFu;
+constexpr uint32_t kNonOpaqueBit = 0x80000000u;
+// 1 + 2*gamma(3) rounded up: far-plane inflation of the robust slab test
+// (Ize 2013, "Robust BVH Ray Traversal", JCGT 2(2)).
+constexpr uint32_t kFarScaleBits = 0x3f800003u;
+
+struct AsHeader {
+    uint32_t magic, kind, node_count, prim_count;
+    uint32_t geom_count, max_depth, node_width, flags;
+    float lo[3];
+    float pad0;
+    float hi[3];
+    float pad1;
+    uint64_t nodes_off, prims_off, extra_off, level_off;
+    uint64_t total_size;
+    uint64_t leafbox_off;  // TLAS: refit scratch of leaf boxes
+    uint64_t reserved[2];
+};
+static_assert(sizeof(AsHeader) == 128, "header");
+
+struct Node {
+    float x[4];
+    float y[4];
+    float z[4];
+    uint32_t child[2];
+    uint32_t pad[2];
+};
+static_assert(sizeof(Node) == 64, "node");
+
+struct TriRec {
+    float v0[3];
+    uint32_t prim;
+    float v1[3];
+    uint32_t geom;
+    float v2[3];
+    uint32_t pad;
+};
+static_assert(sizeof(TriRec) == 48, "tri");
+
+struct InstRec {
+    float w2o[12];
+    uint64_t blas;
+    uint32_t index;
+    uint32_t mask_flags;
+};
+static_assert(sizeof(InstRec) == 64, "inst");
+
+struct Box {
+    float lo[3], hi[3];
+};
+
+// ---------------------------------------------------------------------------
+// Scalar math (operation order fixed; see the file comment)
+// ---------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+};
+struct V2 {
+    float x, y;
+};
+__host__ __device__ inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__host__ __device__ inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ inline V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__host__ __device__ inline float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__host__ __device__ inline V3 cross(V3 a, V3 b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// normalize(v) = v * (1 / sqrt(dot(v, v)))
+__device__ inline V3 normalize(V3 v) {
+    float s = 1.0f / sqrtf(dot(v, v));
+    return v * s;
+}
+// interpolate(a, b, c, bc) = (a*bc.x + b*bc.y) + c*bc.z  (hit_shader_common.glsl:79-81)
+__device__ inline V3 interp3(V3 a, V3 b, V3 c, V3 bc) { return (a * bc.x + b * bc.y) + c * bc.z; }
+__device__ inline V2 interp2(V2 a, V2 b, V2 c, V3 bc) {
+    return V2{(a.x * bc.x + b.x * bc.y) + c.x * bc.z, (a.y * bc.x + b.y * bc.y) + c.y * bc.z};
+}
+__device__ inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ inline float fractf(float x) { return x - floorf(x); }
+
+// Row-major 3x4 affine: ((m0*x + m1*y) + m2*z) + m3
+__device__ inline V3 m34_point(const float* m, V3 p) {
+    return v3(((m[0] * p.x + m[1] * p.y) + m[2] * p.z) + m[3], ((m[4] * p.x + m[5] * p.y) + m[6] * p.z) + m[7],
+              ((m[8] * p.x + m[9] * p.y) + m[10] * p.z) + m[11]);
+}
+__device__ inline V3 m34_dir(const float* m, V3 d) {
+    return v3((m[0] * d.x + m[1] * d.y) + m[2] * d.z, (m[4] * d.x + m[5] * d.y) + m[6] * d.z,
+              (m[8] * d.x + m[9] * d.y) + m[10] * d.z);
+}
+// mat3(gl_WorldToObject3x4EXT) * n with m the row-major world-to-object matrix
+__device__ inline V3 m34_tdir(const float* m, V3 n) {
+    return v3((m[0] * n.x + m[4] * n.y) + m[8] * n.z, (m[1] * n.x + m[5] * n.y) + m[9] * n.z,
+              (m[2] * n.x + m[6] * n.y) + m[10] * n.z);
+}
+// Inverse of a row-major 3x4 affine transform by cofactors.
+__device__ inline void m34_inverse(const float* m, float* inv) {
+    float a = m[0], b = m[1], c = m[2];
+    float d = m[4], e = m[5], f = m[6];
+    float g = m[8], h = m[9], i = m[10];
+    float c00 = e * i - f * h;
+    float c01 = f * g - d * i;
+    float c02 = d * h - e * g;
+    float det = (a * c00 + b * c01) + c * c02;
+    float id = 1.0f / det;
+    inv[0] = c00 * id;
+    inv[1] = (c * h - b * i) * id;
+    inv[2] = (b * f - c * e) * id;
+    inv[4] = c01 * id;
+    inv[5] = (a * i - c * g) * id;
+    inv[6] = (c * d - a * f) * id;
+    inv[8] = c02 * id;
+    inv[9] = (b * g - a * h) * id;
+    inv[10] = (a * e - b * d) * id;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+        inv[4 * r + 3] = -((inv[4 * r + 0] * m[3] + inv[4 * r + 1] * m[7]) + inv[4 * r + 2] * m[11]);
+}
+// Column-major 4x4 times vec4: ((c0*x + c1*y) + c2*z) + c3*w
+__device__ inline void mat4_mul(const float* m, const float* v, float* out) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[r] = ((m[0 + r] * v[0] + m[4 + r] * v[1]) + m[8 + r] * v[2]) + m[12 + r] * v[3];
+}
+
+__device__ inline float u2f(uint32_t u) { return __uint_as_float(u); }
+__device__ inline uint32_t f2u(float f) { return __float_as_uint(f); }
+
+constexpr float kPi = 3.14159265358979323846f;
+
+// sin and cos: Cody-Waite reduction by pi/2 (parts of 12, 12 and 24 bits),
+// Taylor series of sin to x^9 and cos to x^10 on [-pi/4, pi/4].
+__device__ inline void sincos_own(float x, float* s_out, float* c_out) {
+    const float two_over_pi = u2f(0x3f22f983u);
+    const float p1 = u2f(0x3fc90000u);
+    const float p2 = u2f(0x39fda000u);
+    const float p3 = u2f(0x33a22169u);
+    float qf = rintf(x * two_over_pi);
+    int q = (int)qf;
+    float r = ((x - qf * p1) - qf * p2) - qf * p3;
+    float r2 = r * r;
+    float sp = r + (r * r2) * (-1.66666667e-1f + r2 * (8.33333333e-3f + r2 * (-1.98412698e-4f + r2 * 2.75573192e-6f)));
+    float cp = 1.0f + r2 * (-0.5f + r2 * (4.16666667e-2f + r2 * (-1.38888889e-3f + r2 * (2.48015873e-5f + r2 * -2.75573192e-7f))));
+    float s, c;
+    switch (q & 3) {
+        case 0: s = sp; c = cp; break;
+        case 1: s = cp; c = -sp; break;
+        case 2: s = -sp; c = -cp; break;
+        default: s = -cp; c = sp; break;
+    }
+    *s_out = s;
+    *c_out = c;
+}
+__device__ inline float cos_own(float x) {
+    float s, c;
+    sincos_own(x, &s, &c);
+    return c;
+}
+
+// log2 via x = m 2^e, m in [sqrt(1/2), sqrt(2)), 2 atanh((m-1)/(m+1)) / ln 2.
+__device__ inline float log2_own(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -__builtin_inff() : __builtin_nanf("");
+    if (x == __builtin_inff()) return x;
+    int e = 0;
+    uint32_t u = f2u(x);
+    if ((u >> 23) == 0) {
+        x = x * 8388608.0f;
+        u = f2u(x);
+        e = -23;
+    }
+    e += (int)((u >> 23) & 0xffu) - 127;
+    float m = u2f((u & 0x007fffffu) | 0x3f800000u);
+    if (m > u2f(0x3fb504f3u)) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float series = 1.0f + s2 * (3.33333333e-1f + s2 * (2.0e-1f + s2 * (1.42857143e-1f + s2 * (1.11111111e-1f + s2 * 9.09090909e-2f))));
+    float ln_m = (2.0f * s) * series;
+    return (float)e + ln_m * u2f(0x3fb8aa3bu);
+}
+// 2^y via y = n + f, Taylor series of e^(f ln 2) to degree 7, two-step scaling.
+__device__ inline float exp2_own(float y) {
+    if (y != y) return y;
+    if (y >= 128.0f) return __builtin_inff();
+    if (y < -150.0f) return 0.0f;
+    float nf = rintf(y);
+    float f = y - nf;
+    float t = f * u2f(0x3f317218u);
+    float p = 1.0f + t * (1.0f + t * (0.5f + t * (1.66666667e-1f + t * (4.16666667e-2f + t * (8.33333333e-3f + t * (1.38888889e-3f + t * 1.98412698e-4f))))));
+    int n = (int)nf;
+    int n1 = n / 2, n2 = n - n1;
+    float s1 = u2f((uint32_t)(n1 + 127) << 23);
+    float s2 = u2f((uint32_t)(n2 + 127) << 23);
+    return (p * s1) * s2;
+}
+__device__ inline float pow_own(float x, float y) {
+    if (x == 0.0f) return y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : __builtin_inff());
+    return exp2_own(y * log2_own(x));
+}
+// pow(x, 5.0) as ((x*x)*(x*x))*x
+__device__ inline float pow5(float x) {
+    float x2 = x * x;
+    return (x2 * x2) * x;
+}
+
+// ---------------------------------------------------------------------------
+// Textures: sampler2D lookups (LOD 0, REPEAT wrap, own filtering).
+// ---------------------------------------------------------------------------
+struct SceneView {
+    const rtg_model_info* models;
+    const rtg_texture* textures;
+    const float* srgb_lut;  // 256 entries (IEC 61966-2-1 decode)
+    uint32_t model_count;
+    uint32_t texture_count;
+};
+
+__device__ inline int wrap_coord(float f, uint32_t size) {
+    if (!(fabsf(f) < 16777216.0f)) return 0;
+    long long i = (long long)f;
+    long long m = i % (long long)size;
+    if (m < 0) m += size;
+    return (int)m;
+}
+
+__device__ inline float4 fetch_texel(const rtg_texture& t, const float* lut, int x, int y) {
+    uint64_t idx = (uint64_t)y * t.width + (uint64_t)x;
+    if (t.format == RTG_FORMAT_RGBA32_FLOAT) return reinterpret_cast<const float4*>(t.data)[idx];
+    uint32_t p = reinterpret_cast<const uint32_t*>(t.data)[idx];
+    uint32_t r = p & 0xffu, g = (p >> 8) & 0xffu, b = (p >> 16) & 0xffu, a = p >> 24;
+    float4 o;
+    if (t.format == RTG_FORMAT_RGBA8_SRGB) {
+        o.x = lut[r];
+        o.y = lut[g];
+        o.z = lut[b];
+    } else {
+        o.x = (float)r / 255.0f;
+        o.y = (float)g / 255.0f;
+        o.z = (float)b / 255.0f;
+    }
+    o.w = (float)a / 255.0f;
+    return o;
+}
+
+__device__ inline float4 sample_tex(const rtg_texture& t, const float* lut, float u, float v) {
+    if (t.width == 0 || t.height == 0) return make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    if (t.filter == RTG_FILTER_NEAREST) {
+        int x = wrap_coord(floorf(u * (float)t.width), t.width);
+        int y = wrap_coord(floorf(v * (float)t.height), t.height);
+        return fetch_texel(t, lut, x, y);
+    }
+    float x = u * (float)t.width - 0.5f;
+    float y = v * (float)t.height - 0.5f;
+    float fx = floorf(x), fy = floorf(y);
+    float a = x - fx, b = y - fy;
+    int x0 = wrap_coord(fx, t.width), y0 = wrap_coord(fy, t.height);
+    int x1 = x0 + 1 == (int)t.width ? 0 : x0 + 1;
+    int y1 = y0 + 1 == (int)t.height ? 0 : y0 + 1;
+    float4 t00 = fetch_texel(t, lut, x0, y0), t10 = fetch_texel(t, lut, x1, y0);
+    float4 t01 = fetch_texel(t, lut, x0, y1), t11 = fetch_texel(t, lut, x1, y1);
+    float ia = 1.0f - a, ib = 1.0f - b;
+    float4 o;
+    o.x = (t00.x * ia + t10.x * a) * ib + (t01.x * ia + t11.x * a) * b;
+    o.y = (t00.y * ia + t10.y * a) * ib + (t01.y * ia + t11.y * a) * b;
+    o.z = (t00.z * ia + t10.z * a) * ib + (t01.z * ia + t11.z * a) * b;
+    o.w = (t00.w * ia + t10.w * a) * ib + (t01.w * ia + t11.w * a) * b;
+    return o;
+}
+
+__device__ inline float4 sample_index(const SceneView& S, uint32_t index, float u, float v) {
+    if (index >= S.texture_count) return make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    rtg_texture t = S.textures[index];
+    return sample_tex(t, S.srgb_lut, u, v);
+}
+
+// ---------------------------------------------------------------------------
+// Rays, triangles, boxes
+// ---------------------------------------------------------------------------
+struct RayCtx {
+    V3 o, d, inv;
+    float sx, sy, sz;
+    int kx, ky, kz;
+};
+
+__device__ inline float sel3(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+__device__ inline void ray_setup(RayCtx& r, V3 o, V3 d) {
+    r.o = o;
+    r.d = d;
+    r.inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    float dz = sel3(d, kz);
+    if (dz < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    r.sx = sel3(d, kx) / dz;
+    r.sy = sel3(d, ky) / dz;
+    r.sz = 1.0f / dz;
+}
+
+// Watertight ray/triangle test (Woop, Benthin, Wald 2013, JCGT 2(1)).
+// Returns true for a hit with tmin <= t <= tmax; (u, v) weight vertices 1, 2.
+__device__ inline bool tri_test(const RayCtx& r, V3 p0, V3 p1, V3 p2, float tmin, float tmax, float& t_out,
+                                float& u_out, float& v_out) {
+    V3 A = p0 - r.o, B = p1 - r.o, C = p2 - r.o;
+    float Akz = sel3(A, r.kz), Bkz = sel3(B, r.kz), Ckz = sel3(C, r.kz);
+    float Ax = sel3(A, r.kx) - r.sx * Akz, Ay = sel3(A, r.ky) - r.sy * Akz;
+    float Bx = sel3(B, r.kx) - r.sx * Bkz, By = sel3(B, r.ky) - r.sy * Bkz;
+    float Cx = sel3(C, r.kx) - r.sx * Ckz, Cy = sel3(C, r.ky) - r.sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = (U + V) + W;
+    if (det == 0.0f) return false;
+    float Az = r.sz * Akz, Bz = r.sz * Bkz, Cz = r.sz * Ckz;
+    float T = (U * Az + V * Bz) + W * Cz;
+    float t = T / det;
+    if (!(t >= tmin && t <= tmax)) return false;
+    t_out = t;
+    u_out = V / det;
+    v_out = W / det;
+    return true;
+}
+
+// Robust slab test of both children of a node (Ize 2013). Near and far planes
+// are chosen by the sign bit of the ray direction.
+__device__ inline void box_test2(const float4& X, const float4& Y, const float4& Z, const RayCtx& r, float tmin,
+                                 float tcur, bool& h0, bool& h1, float& tn0, float& tn1) {
+    const bool sx = signbit(r.d.x), sy = signbit(r.d.y), sz = signbit(r.d.z);
+    const float far_scale = u2f(kFarScaleBits);
+    {
+        float nx = sx ? X.y : X.x, fx = sx ? X.x : X.y;
+        float ny = sy ? Y.y : Y.x, fy = sy ? Y.x : Y.y;
+        float nz = sz ? Z.y : Z.x, fz = sz ? Z.x : Z.y;
+        float tnx = (nx - r.o.x) * r.inv.x, tfx = (fx - r.o.x) * r.inv.x;
+        float tny = (ny - r.o.y) * r.inv.y, tfy = (fy - r.o.y) * r.inv.y;
+        float tnz = (nz - r.o.z) * r.inv.z, tfz = (fz - r.o.z) * r.inv.z;
+        float tnear = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
+        float tfar = fminf(fminf(fminf(tfx, tfy), tfz) * far_scale, tcur);
+        tn0 = tnear;
+        h0 = tnear <= tfar;
+    }
+    {
+        float nx = sx ? X.w : X.z, fx = sx ? X.z : X.w;
+        float ny = sy ? Y.w : Y.z, fy = sy ? Y.z : Y.w;
+        float nz = sz ? Z.w : Z.z, fz = sz ? Z.z : Z.w;
+        float tnx = (nx - r.o.x) * r.inv.x, tfx = (fx - r.o.x) * r.inv.x;
+        float tny = (ny - r.o.y) * r.inv.y, tfy = (fy - r.o.y) * r.inv.y;
+        float tnz = (nz - r.o.z) * r.inv.z, tfz = (fz - r.o.z) * r.inv.z;
+        float tnear = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
+        float tfar = fminf(fminf(fminf(tfx, tfy), tfz) * far_scale, tcur);
+        tn1 = tnear;
+        h1 = tnear <= tfar;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Two-level traversal
+// ---------------------------------------------------------------------------
+struct HitRec {
+    float t, u, v;
+    uint32_t inst, prim, geom;
+};
+
+struct TravCounters {
+    uint32_t nodes, tris, insts, anyhits;
+};
+
+struct TlasView {
+    const uint8_t* base;
+    const Node* nodes;
+    const InstRec* recs;
+    const rtg_instance* src;
+};
+
+__device__ inline TlasView tlas_view(const uint8_t* base) {
+    const AsHeader* h = reinterpret_cast<const AsHeader*>(base);
+    TlasView v;
+    v.base = base;
+    v.nodes = reinterpret_cast<const Node*>(base + h->nodes_off);
+    v.recs = reinterpret_cast<const InstRec*>(base + h->prims_off);
+    v.src = reinterpret_cast<const rtg_instance*>(base + h->extra_off);
+    return v;
+}
+
+__device__ inline bool key_less(uint32_t ia, uint32_t ga, uint32_t pa, uint32_t ib, uint32_t gb, uint32_t pb) {
+    if (ia != ib) return ia < ib;
+    if (ga != gb) return ga < gb;
+    return pa < pb;
+}
+
+__device__ inline bool is_opaque(uint32_t geom_word, uint32_t inst_flags, uint32_t ray_flags) {
+    bool opaque = (geom_word & kNonOpaqueBit) == 0;
+    if (inst_flags & RTG_INSTANCE_FORCE_OPAQUE) opaque = true;
+    else if (inst_flags & RTG_INSTANCE_FORCE_NO_OPAQUE) opaque = false;
+    if (ray_flags & RTG_RAY_FLAG_OPAQUE) opaque = true;
+    else if (ray_flags & RTG_RAY_FLAG_NO_OPAQUE) opaque = false;
+    return opaque;
+}
+
+// any_hit_alpha_clip (shaders/any_hit_alpha_clip.glsl:11-29): keep iff alpha >= 0.5
+__device__ inline bool alpha_clip_keep(const SceneView& S, const rtg_instance* src, uint32_t inst_index, uint32_t geom,
+                                       uint32_t prim, float u, float v) {
+    if (S.models == nullptr) return true;
+    uint32_t custom = src[inst_index].custom_index_and_mask & 0xffffffu;
+    if (custom >= S.model_count) return true;
+    const rtg_model_info mi = S.models[custom];
+    const rtg_geometry_info* gi = reinterpret_cast<const rtg_geometry_info*>(mi.geometry_info_address) + geom;
+    const uint32_t* idx = reinterpret_cast<const uint32_t*>(gi->index_buffer_address) + 3ull * prim;
+    const float* uvs = reinterpret_cast<const float*>(mi.uv_buffer_address);
+    V3 bc = v3((1.0f - u) - v, u, v);
+    uint32_t i0 = idx[0], i1 = idx[1], i2 = idx[2];
+    V2 a{uvs[2 * i0], uvs[2 * i0 + 1]}, b{uvs[2 * i1], uvs[2 * i1 + 1]}, c{uvs[2 * i2], uvs[2 * i2 + 1]};
+    V2 uv = interp2(a, b, c, bc);
+    float4 tx = sample_index(S, gi->diffuse_image_index, uv.x, uv.y);
+    return !(tx.w < 0.5f);
+}
+
+// Per-lane traversal stack: the first kLdsDepth entries live in LDS
+// (entry-major so that lanes never conflict), deeper entries in a per-thread
+// global spill area of kSpillDepth entries.
+constexpr int kLdsDepth = 16;
+constexpr int kSpillDepth = 48;
+constexpr int kStackMax = kLdsDepth + kSpillDepth;
+
+struct Stack {
+    uint2* lds;      // lds[k * stride + lane]
+    uint2* spill;    // spill[k - kLdsDepth], per thread
+    int stride;
+    int sp;
+    __device__ inline void push(uint32_t ref, float tn) {
+        uint2 e = make_uint2(ref, f2u(tn));
+        if (sp < kLdsDepth) lds[sp * stride] = e;
+        else spill[sp - kLdsDepth] = e;
+        ++sp;
+    }
+    __device__ inline uint2 pop() {
+        --sp;
+        return sp < kLdsDepth ? lds[sp * stride] : spill[sp - kLdsDepth];
+    }
+};
+
+// Returns 1 for an accepted hit, 0 for none, -1 if the stack overflowed.
+template <bool OCCLUSION, bool COUNT>
+__device__ inline int traverse(const TlasView& T, const SceneView& S, V3 o, V3 d, float tmin, float tmax,
+                               uint32_t ray_flags, uint32_t cull_mask, uint32_t sbt_offset, HitRec& best, Stack& st,
+                               TravCounters& cnt) {
+    RayCtx world, obj;
+    ray_setup(world, o, d);
+    bool in_blas = false;
+    const Node* nodes = T.nodes;
+    const TriRec* tris = nullptr;
+    uint32_t inst_index = 0, inst_flags = 0;
+    float tcur = tmax;
+    int found = 0;
+    best.t = tmax;
+    best.u = best.v = 0.0f;
+    best.inst = best.prim = best.geom = RTG_NO_HIT;
+    st.sp = 0;
+    uint32_t cur = 0;
+    bool have = true;
+    while (true) {
+        if (have) {
+            const RayCtx& cr = in_blas ? obj : world;
+            if (!(cur & kLeafBit)) {
+                const float4* np = reinterpret_cast<const float4*>(nodes + cur);
+                float4 X = np[0], Y = np[1], Z = np[2];
+                uint4 C = reinterpret_cast<const uint4*>(np)[3];
+                if (COUNT) cnt.nodes++;
+                bool h0, h1;
+                float tn0, tn1;
+                box_test2(X, Y, Z, cr, tmin, tcur, h0, h1, tn0, tn1);
+                h0 = h0 && C.x != kEmpty;
+                h1 = h1 && C.y != kEmpty;
+                if (h0 && h1) {
+                    bool first1 = tn1 < tn0;
+                    cur = first1 ? C.y : C.x;
+                    if (st.sp >= kStackMax) return -1;
+                    st.push(first1 ? C.x : C.y, first1 ? tn0 : tn1);
+                } else if (h0) {
+                    cur = C.x;
+                } else if (h1) {
+                    cur = C.y;
+                } else {
+                    have = false;
+                }
+                continue;
+            }
+            if (!in_blas) {
+                const InstRec* rec = T.recs + (cur & kFirstMask);
+                have = false;
+                uint32_t mf = rec->mask_flags;
+                if (((mf & 0xffu) & cull_mask) == 0) continue;
+                const uint8_t* bb = reinterpret_cast<const uint8_t*>(rec->blas);
+                if (bb == nullptr) continue;
+                if (COUNT) cnt.insts++;
+                if (st.sp >= kStackMax) return -1;
+                st.push(kSentinel, -__builtin_inff());
+                float w2o[12];
+#pragma unroll
+                for (int k = 0; k < 12; ++k) w2o[k] = rec->w2o[k];
+                ray_setup(obj, m34_point(w2o, o), m34_dir(w2o, d));
+                const AsHeader* bh = reinterpret_cast<const AsHeader*>(bb);
+                nodes = reinterpret_cast<const Node*>(bb + bh->nodes_off);
+                tris = reinterpret_cast<const TriRec*>(bb + bh->prims_off);
+                inst_index = rec->index;
+                inst_flags = (mf >> 8) & 0xffu;
+                in_blas = true;
+                cur = 0;
+                have = true;
+                continue;
+            }
+            uint32_t first = cur & kFirstMask;
+            uint32_t count = ((cur >> 27) & 0xfu) + 1u;
+            for (uint32_t k = first; k < first + count; ++k) {
+                const float4* tp = reinterpret_cast<const float4*>(tris + k);
+                float4 a = tp[0], b = tp[1], c = tp[2];
+                if (COUNT) cnt.tris++;
+                float t, u, v;
+                if (!tri_test(obj, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), tmin, tcur, t, u, v))
+                    continue;
+                uint32_t prim = f2u(a.w), gword = f2u(b.w);
+                uint32_t geom = gword & ~kNonOpaqueBit;
+                if (!OCCLUSION &&
+                    !(t < best.t || (t == best.t && key_less(inst_index, geom, prim, best.inst, best.geom, best.prim))))
+                    continue;
+                if (!is_opaque(gword, inst_flags, ray_flags)) {
+                    uint32_t group = (T.src[inst_index].sbt_offset_and_flags & 0xffffffu) + sbt_offset;
+                    if (group < RTG_HIT_GROUP_COUNT) {
+                        if (COUNT) cnt.anyhits++;
+                        if (!alpha_clip_keep(S, T.src, inst_index, geom, prim, u, v)) continue;
+                    }
+                }
+                found = 1;
+                best.t = t;
+                best.u = u;
+                best.v = v;
+                best.inst = inst_index;
+                best.geom = geom;
+                best.prim = prim;
+                if (OCCLUSION || (ray_flags & RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT)) return 1;
+                tcur = t;
+            }
+            have = false;
+            continue;
+        }
+        // pop
+        while (st.sp > 0) {
+            uint2 e = st.pop();
+            if (e.x == kSentinel) {
+                in_blas = false;
+                nodes = T.nodes;
+                continue;
+            }
+            if (u2f(e.y) > tcur) continue;
+            cur = e.x;
+            have = true;
+            break;
+        }
+        if (!have) break;
+    }
+    return found;
+}
+
+}  // namespace rtg

@@ -1,0 +1,118 @@
+// rtg_internal.hpp -- host-side state of an rtg context.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/rtg.h"
+
+namespace rtg {
+
+void set_error(const char* fmt, ...);
+void clear_error();
+
+#define RTG_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            ::rtg::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                             __LINE__);                                                    \
+            return RTG_ERROR_DEVICE;                                                       \
+        }                                                                                  \
+    } while (0)
+
+#define RTG_REQUIRE(cond, code, ...)       \
+    do {                                   \
+        if (!(cond)) {                     \
+            ::rtg::set_error(__VA_ARGS__); \
+            return (code);                 \
+        }                                  \
+    } while (0)
+
+struct AsRecord {
+    uint32_t kind;
+    uint32_t prim_count;
+    uint32_t node_count;
+    uint64_t size;
+};
+
+// Grow-only device buffer.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t size = 0;
+    int ensure(size_t bytes);
+    void release();
+};
+
+// Bump allocator over a DeviceBuffer (256-byte aligned slices).
+struct Carve {
+    uint8_t* base;
+    size_t off = 0;
+    explicit Carve(void* b) : base(static_cast<uint8_t*>(b)) {}
+    template <class T>
+    T* take(size_t count) {
+        off = (off + 255) & ~size_t(255);
+        T* p = reinterpret_cast<T*>(base + off);
+        off += count * sizeof(T);
+        return p;
+    }
+};
+template <class T>
+inline void carve_size(size_t& off, size_t count) {
+    off = (off + 255) & ~size_t(255);
+    off += count * sizeof(T);
+}
+
+enum StatusBits : uint32_t {
+    kStatusBadIndex = 1u << 0,      // BLAS build: index > max_vertex
+    kStatusBadBlas = 1u << 1,       // TLAS build/refit: reference is not a live BLAS
+    kStatusStackOverflow = 1u << 2, // trace: traversal stack exhausted
+};
+
+struct TimedLaunch {
+    uint32_t kind;
+    hipEvent_t start, stop;
+};
+
+struct FrameArena;  // trace.hip
+
+}  // namespace rtg
+
+struct rtg_ctx {
+    int device = 0;
+    int cu_count = 0;
+    std::map<uint64_t, rtg::AsRecord> structures;
+    // sorted addresses of live BLASes, on the device (TLAS build/refit validate against it)
+    uint64_t* d_registry = nullptr;
+    uint32_t registry_count = 0;
+    std::vector<void*> retired;  // old registry buffers (freed at destroy)
+    rtg::DeviceBuffer scratch;   // build scratch arena
+    float* d_srgb_lut = nullptr;
+    uint32_t* d_status = nullptr;  // sticky kernel status bits
+    uint64_t* d_counters = nullptr;  // rtg_trace_counters on the device
+    bool counters_enabled = false;
+    bool timing = false;
+    std::vector<rtg::TimedLaunch> pending;
+    double kernel_ms[RTG_KERNEL_COUNT] = {};
+    uint64_t kernel_launches[RTG_KERNEL_COUNT] = {};
+    rtg::FrameArena* frame = nullptr;
+};
+
+namespace rtg {
+int refresh_registry(rtg_ctx* ctx);
+// Event pair around a launch when timing is enabled.
+struct LaunchTimer {
+    rtg_ctx* ctx;
+    uint32_t kind;
+    hipStream_t stream;
+    hipEvent_t a = nullptr, b = nullptr;
+    LaunchTimer(rtg_ctx* c, uint32_t k, hipStream_t s);
+    ~LaunchTimer();
+};
+int check_status(rtg_ctx* ctx, hipStream_t stream, uint32_t mask, const char* what);
+void free_frame_arena(rtg_ctx* ctx);
+}  // namespace rtg

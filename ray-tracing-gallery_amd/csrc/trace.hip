@@ -1,0 +1,882 @@
+// trace.hip -- the trace step on gfx950: ray queries and the wavefront
+// pipeline behind rtg_trace_image / rtg_trace_batch.
+//
+// vkCmdTraceRaysKHR(W, H, 1) (src/command_buffer_recording.rs:116-126) runs
+// ray_generation once per pixel; the pixel's bounce loop (up to 3 traces,
+// shaders/ray-tracing/src/lib.rs:144-172) calls the hit and miss stages, and
+// closest_hit_textured traces its shadow rays (closest_hit_textured.glsl:195).
+// Here each stage is a kernel over a queue of rays:
+//   raygen -> [trace closest -> shade] x max_traces -> trace occlusion -> resolve
+// Shade appends continuation rays (mirror, portal) and shadow rays to queues
+// with one atomic per wavefront (ballot/prefix-sum compaction). A path's
+// result depends only on its own rays, so queue order never changes results.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "rtg_device.hpp"
+#include "rtg_internal.hpp"
+
+namespace rtg {
+
+constexpr int kTraceThreads = 256;
+
+struct JobDev {
+    const uint8_t* tlas;
+    SceneView scene;
+    float view_inverse[16];
+    float proj_inverse[16];
+    float sun[3];
+    float sun_radius;
+    uint32_t bn_index;
+    uint32_t frame_index;
+    uint32_t width, height, spp, shadow_rays, max_traces, flags;
+    uint64_t sample_base;  // first path of the job
+    uint64_t pixel_base;   // first pixel of the job in the batch's pixel numbering
+    uint32_t* out_rgba8;
+    float* out_rgb;
+    rtg_hit* out_hits;
+};
+
+struct Queue {
+    float4* o;  // origin.xyz, tmin
+    float4* d;  // direction.xyz, tmax
+    uint32_t* path;
+};
+
+struct Frame {
+    const JobDev* jobs;
+    uint32_t njobs;
+    uint64_t paths;
+    float4* path_rgb;    // final colour of paths that did not end on a textured hit
+    float4* shade_a;     // NoL, combined.xyz of a textured hit
+    float4* shade_b;     // ambient.xyz, 1 if textured
+    uint32_t* vis;       // visible shadow rays
+    uint32_t* cycles;    // heatmap cycle counts
+    Queue q[2];
+    uint4* hit_a;        // t bits, instance, primitive, geometry
+    float2* hit_b;       // u, v
+    Queue sq;            // shadow rays
+    uint32_t* counts;    // [0..3] trace queue sizes per trace level, [4] shadow rays
+    uint64_t* counters;  // rtg_trace_counters
+    uint2* spill;
+};
+
+__device__ inline uint32_t find_job(const JobDev* jobs, uint32_t njobs, uint64_t p, bool by_pixel) {
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        uint64_t b = by_pixel ? jobs[mid].pixel_base : jobs[mid].sample_base;
+        if (b <= p) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Wavefront-aggregated allocation: every lane of the wave calls it (lanes
+// that need nothing pass 0); one atomic per wave.
+__device__ inline uint32_t wave_alloc(uint32_t want, uint32_t* counter) {
+    const int lane = threadIdx.x & 63;
+    uint32_t incl = want;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t v = (uint32_t)__shfl_up((int)incl, off);
+        if (lane >= off) incl += v;
+    }
+    uint32_t total = (uint32_t)__shfl((int)incl, 63);
+    uint32_t base = 0;
+    if (lane == 63 && total) base = atomicAdd(counter, total);
+    base = (uint32_t)__shfl((int)base, 63);
+    return base + incl - want;
+}
+
+__device__ inline void wave_add_counter(uint64_t* c, uint32_t v) {
+    uint64_t x = v;
+    for (int off = 32; off > 0; off >>= 1) x += (uint64_t)__shfl_xor((long long)x, off);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd((unsigned long long*)c, (unsigned long long)x);
+}
+
+// Traversal counters of rtg_trace_counters: slots 3..6 for closest-hit
+// queries, 7..10 for occlusion queries.
+constexpr int kCountersClosest = 3;
+constexpr int kCountersOcclusion = 7;
+constexpr int kCounterSlots = 11;
+
+__device__ inline void flush_counters(uint64_t* counters, const TravCounters& c, int base) {
+    wave_add_counter(&counters[base + 0], c.nodes);
+    wave_add_counter(&counters[base + 1], c.tris);
+    wave_add_counter(&counters[base + 2], c.insts);
+    wave_add_counter(&counters[base + 3], c.anyhits);
+}
+
+// ---------------------------------------------------------------------------
+// Ray queries: rtg_trace_rays
+// ---------------------------------------------------------------------------
+template <bool OCC, bool COUNT>
+__global__ __launch_bounds__(kTraceThreads) void k_trace_rays(const uint8_t* tlas, SceneView S, const rtg_ray* rays,
+                                                              uint32_t n, uint32_t ray_flags, uint32_t cull_mask,
+                                                              uint32_t sbt_offset, rtg_hit* hits, uint32_t* occ,
+                                                              uint2* spill, uint64_t* counters, uint32_t* status) {
+    __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
+    const TlasView T = tlas_view(tlas);
+    const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
+    Stack st;
+    st.lds = s_stack + threadIdx.x;
+    st.stride = kTraceThreads;
+    st.spill = spill + (size_t)gtid * kSpillDepth;
+    TravCounters cnt = {0, 0, 0, 0};
+    const uint32_t stride = gridDim.x * kTraceThreads;
+    const uint32_t rounds = (n + stride - 1) / stride;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        uint32_t i = r * stride + gtid;
+        if (i < n) {
+            const float4* rp = reinterpret_cast<const float4*>(rays + i);
+            float4 a = rp[0], b = rp[1];
+            HitRec h;
+            int res = traverse<OCC, COUNT>(T, S, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, ray_flags, cull_mask,
+                                           sbt_offset, h, st, cnt);
+            if (res < 0) atomicOr(status, (uint32_t)kStatusStackOverflow);
+            if (OCC) {
+                occ[i] = res > 0 ? 1u : 0u;
+            } else {
+                rtg_hit o;
+                o.t = h.t;
+                o.instance = h.inst;
+                o.primitive = h.prim;
+                o.geometry = h.geom;
+                o.u = h.u;
+                o.v = h.v;
+                hits[i] = o;
+            }
+        }
+    }
+    if (COUNT) flush_counters(counters, cnt, OCC ? kCountersOcclusion : kCountersClosest);
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront pipeline
+// ---------------------------------------------------------------------------
+__global__ void k_frame_begin(uint32_t* counts, uint64_t* counters, bool reset_counters) {
+    if (threadIdx.x < 8) counts[threadIdx.x] = 0;
+    if (reset_counters && threadIdx.x < kCounterSlots) counters[threadIdx.x] = 0;
+}
+
+__global__ void k_reset_counters(uint64_t* counters, uint64_t primary) {
+    if (threadIdx.x < kCounterSlots) counters[threadIdx.x] = threadIdx.x == 0 ? primary : 0;
+}
+
+// ray_generation (shaders/ray-tracing/src/lib.rs:94-142): one path per
+// (pixel, sample). Sample s > 0 of a multi-sample pixel moves the pixel
+// centre by the R2 sequence; sample 0 of one sample is the reference's +0.5.
+__global__ __launch_bounds__(256) void k_raygen(Frame F, bool count) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < F.paths; p += stride) {
+        uint32_t j = find_job(F.jobs, F.njobs, p, false);
+        const JobDev& J = F.jobs[j];
+        uint64_t local = p - J.sample_base;
+        uint32_t s = (uint32_t)(local % J.spp);
+        uint32_t pix = (uint32_t)(local / J.spp);
+        uint32_t px = pix % J.width, py = pix / J.width;
+        float jx = 0.5f, jy = 0.5f;
+        if (J.spp > 1) {
+            jx = fractf(0.5f + (float)s * 0.754877669f);
+            jy = fractf(0.5f + (float)s * 0.569840296f);
+        }
+        float pcx = (float)px + jx, pcy = (float)py + jy;
+        float tcx = pcx / (float)J.width, tcy = pcy / (float)J.height;
+        float rcx = tcx * 2.0f - 1.0f, rcy = tcy * 2.0f - 1.0f;
+        float zo[4] = {0.0f, 0.0f, 0.0f, 1.0f}, tmp[4];
+        mat4_mul(J.view_inverse, zo, tmp);
+        V3 origin = v3(tmp[0], tmp[1], tmp[2]);
+        float tv[4] = {rcx, rcy, 1.0f, 1.0f};
+        mat4_mul(J.proj_inverse, tv, tmp);
+        V3 ld = normalize(v3(tmp[0], tmp[1], tmp[2]));
+        float dv[4] = {ld.x, ld.y, ld.z, 0.0f};
+        mat4_mul(J.view_inverse, dv, tmp);
+        F.q[0].o[p] = make_float4(origin.x, origin.y, origin.z, 0.01f);
+        F.q[0].d[p] = make_float4(tmp[0], tmp[1], tmp[2], 10000.0f);
+        F.q[0].path[p] = (uint32_t)p;
+        F.path_rgb[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        F.shade_b[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        F.vis[p] = 0;
+        F.cycles[p] = 0;
+    }
+    if (count && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long*)&F.counters[0], F.paths);
+}
+
+// Closest-hit traversal of trace level b (count from the device unless fixed).
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int level, const uint32_t* count_ptr,
+                                                                 uint32_t fixed_count, uint32_t* status) {
+    __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
+    const uint32_t n = count_ptr ? *count_ptr : fixed_count;
+    const Queue Q = F.q[level & 1];
+    const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
+    Stack st;
+    st.lds = s_stack + threadIdx.x;
+    st.stride = kTraceThreads;
+    st.spill = F.spill + (size_t)gtid * kSpillDepth;
+    TravCounters cnt = {0, 0, 0, 0};
+    const uint32_t stride = gridDim.x * kTraceThreads;
+    const uint32_t rounds = (n + stride - 1) / stride;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        uint32_t i = r * stride + gtid;
+        if (i < n) {
+            uint32_t p = Q.path[i];
+            uint32_t j = find_job(F.jobs, F.njobs, p, false);
+            const JobDev& J = F.jobs[j];
+            const TlasView T = tlas_view(J.tlas);
+            float4 a = Q.o[i], b = Q.d[i];
+            bool heat = (J.flags & RTG_TRACE_HEATMAP) != 0;
+            uint64_t c0 = heat ? __builtin_readcyclecounter() : 0;
+            HitRec h;
+            int res = traverse<false, COUNT>(T, J.scene, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w,
+                                             RTG_RAY_FLAG_NONE, 0xffu, 0u, h, st, cnt);
+            if (heat) atomicAdd(&F.cycles[p], (uint32_t)(__builtin_readcyclecounter() - c0));
+            if (res < 0) atomicOr(status, (uint32_t)kStatusStackOverflow);
+            F.hit_a[i] = make_uint4(f2u(h.t), h.inst, h.prim, h.geom);
+            F.hit_b[i] = make_float2(h.u, h.v);
+        }
+    }
+    if (COUNT) flush_counters(F.counters, cnt, kCountersClosest);
+}
+
+__device__ inline V3 ld3(uint64_t base, uint32_t i) {
+    const float* p = reinterpret_cast<const float*>(base) + 3ull * i;
+    return v3(p[0], p[1], p[2]);
+}
+__device__ inline V2 ld2(uint64_t base, uint32_t i) {
+    const float* p = reinterpret_cast<const float*>(base) + 2ull * i;
+    return V2{p[0], p[1]};
+}
+
+// sample_directional_light (closest_hit_textured.glsl:77-94)
+__device__ inline V3 sample_dir_light(float rx, float ry, V3 c, float radius) {
+    float rad = sqrtf(rx);
+    float angle = (ry * 2.0f) * kPi;
+    float s, co;
+    sincos_own(angle, &s, &co);
+    float px = (rad * co) * radius, py = (rad * s) * radius;
+    V3 t = normalize(cross(c, v3(0.0f, 1.0f, 0.0f)));
+    V3 b = normalize(cross(t, c));
+    return normalize((c + t * px) + b * py);
+}
+
+// pbr.glsl:174-211 (brdf without the light intensity factor): returns NoL
+// and the combined diffuse + specular factor.
+__device__ inline void brdf_parts(V3 n, V3 view, V3 l, float pr, V3 base, float metallic, float reflectance,
+                                  float& NoL_out, V3& comb) {
+    V3 h = normalize(view + l);
+    float rough = pr * pr;
+    float NoV = clampf(dot(n, view), 1e-9f, 1.0f);
+    float NoH = clampf(dot(n, h), 0.0f, 1.0f);
+    float NoL = clampf(dot(n, l), 0.0f, 1.0f);
+    float LoH = clampf(dot(l, h), 0.0f, 1.0f);
+    float a = NoH * rough;
+    float k = rough / ((1.0f - NoH * NoH) + a * a);
+    float D = (k * k) * (1.0f / kPi);
+    float dielectric_f0 = (0.16f * reflectance) * reflectance;
+    float fd0 = dielectric_f0 * (1.0f - metallic);
+    V3 f0 = v3(fd0, fd0, fd0) + base * metallic;
+    float f90 = 0.5f + ((2.0f * rough) * LoH) * LoH;
+    float p5 = pow5(1.0f - LoH);
+    V3 F = v3(f0.x + (f90 - f0.x) * p5, f0.y + (f90 - f0.y) * p5, f0.z + (f90 - f0.z) * p5);
+    float a2 = rough * rough;
+    float ggxv = NoL * sqrtf((NoV * NoV) * (1.0f - a2) + a2);
+    float ggxl = NoV * sqrtf((NoL * NoL) * (1.0f - a2) + a2);
+    float Vis = 0.5f / (ggxv + ggxl);
+    float DV = D * Vis;
+    V3 spec = F * DV;
+    float ls = 1.0f + (f90 - 1.0f) * pow5(1.0f - NoL);
+    float vs = 1.0f + (f90 - 1.0f) * pow5(1.0f - NoV);
+    float fd = (ls * vs) * (1.0f / kPi);
+    V3 diffuse = base * fd;
+    comb = diffuse + spec;
+    NoL_out = NoL;
+}
+
+// Hit and miss stages of trace level b.
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_t* count_ptr, uint32_t fixed_count) {
+    const uint32_t n = count_ptr ? *count_ptr : fixed_count;
+    const Queue Q = F.q[level & 1];
+    const Queue NQ = F.q[(level + 1) & 1];
+    uint32_t* next_count = &F.counts[level + 1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (n + stride - 1) / stride;
+    uint32_t n_shadow = 0, n_bounce = 0;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        uint32_t i = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        bool active = i < n;
+        uint32_t want_shadow = 0;
+        bool want_next = false;
+        V3 sh_origin = v3(0, 0, 0), next_o = v3(0, 0, 0), next_d = v3(0, 0, 0);
+        V3 sun = v3(0, 0, 0);
+        float sun_radius = 0.0f;
+        uint32_t p = 0, px = 0, py = 0, frame = 0, bn_index = 0;
+        const JobDev* J = nullptr;
+        if (active) {
+            p = Q.path[i];
+            J = &F.jobs[find_job(F.jobs, F.njobs, p, false)];
+            uint64_t local = p - J->sample_base;
+            uint32_t pix = (uint32_t)(local / J->spp);
+            uint32_t s = (uint32_t)(local % J->spp);
+            px = pix % J->width;
+            py = pix / J->width;
+            frame = J->frame_index * J->spp + s;
+            sun = v3(J->sun[0], J->sun[1], J->sun[2]);
+            sun_radius = J->sun_radius;
+            bn_index = J->bn_index;
+            uint4 ha = F.hit_a[i];
+            float2 hb = F.hit_b[i];
+            float t = u2f(ha.x);
+            uint32_t inst = ha.y, prim = ha.z, geom = ha.w;
+            float4 ro = Q.o[i], rd = Q.d[i];
+            V3 o = v3(ro.x, ro.y, ro.z), d = v3(rd.x, rd.y, rd.z);
+            if (level == 0 && J->out_hits) {
+                rtg_hit oh;
+                oh.t = t;
+                oh.instance = inst;
+                oh.primitive = prim;
+                oh.geometry = geom;
+                oh.u = hb.x;
+                oh.v = hb.y;
+                J->out_hits[local] = oh;
+            }
+            const SceneView& S = J->scene;
+            if (inst == RTG_NO_HIT) {
+                // primary_ray_miss (lib.rs:40-51)
+                float cs = cos_own(sun_radius);
+                V3 c = dot(d, sun) > cs ? v3(1.0f, 1.0f, 1.0f) : v3(0.0f, 0.0f, 0.05f);
+                F.path_rgb[p] = make_float4(c.x, c.y, c.z, 0.0f);
+            } else {
+                const TlasView T = tlas_view(J->tlas);
+                const rtg_instance& src = T.src[inst];
+                uint32_t group = src.sbt_offset_and_flags & 0xffffffu;
+                uint32_t custom = src.custom_index_and_mask & 0xffffffu;
+                float o2w[12];
+#pragma unroll
+                for (int k = 0; k < 12; ++k) o2w[k] = (&src.transform[0][0])[k];
+                V3 bc = v3((1.0f - hb.x) - hb.y, hb.x, hb.y);
+                if ((group == RTG_HIT_TEXTURED || group == RTG_HIT_MIRROR) && custom >= S.model_count) {
+                    // no model: colour stays 0, no continuation
+                } else if (group == RTG_HIT_TEXTURED) {
+                    // closest_hit_textured (closest_hit_textured.glsl:174-226)
+                    const rtg_model_info mi = S.models[custom];
+                    const rtg_geometry_info gi = reinterpret_cast<const rtg_geometry_info*>(mi.geometry_info_address)[geom];
+                    const uint32_t* idx = reinterpret_cast<const uint32_t*>(gi.index_buffer_address) + 3ull * prim;
+                    uint32_t i0 = idx[0], i1 = idx[1], i2 = idx[2];
+                    V3 pa = ld3(mi.position_buffer_address, i0), pb = ld3(mi.position_buffer_address, i1),
+                       pc = ld3(mi.position_buffer_address, i2);
+                    V3 na = ld3(mi.normal_buffer_address, i0), nb = ld3(mi.normal_buffer_address, i1),
+                       nc = ld3(mi.normal_buffer_address, i2);
+                    V2 ua = ld2(mi.uv_buffer_address, i0), ub = ld2(mi.uv_buffer_address, i1),
+                       uc = ld2(mi.uv_buffer_address, i2);
+                    V3 pos = interp3(pa, pb, pc, bc);
+                    V3 nrm = interp3(na, nb, nc, bc);
+                    V2 uv = interp2(ua, ub, uc, bc);
+                    // get_shadow_terminator_fix_shadow_origin (:13-39)
+                    V3 off[3];
+                    V3 P[3] = {pa, pb, pc}, N[3] = {na, nb, nc};
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        V3 vtp = pos - P[k];
+                        float dp = fminf(0.0f, dot(vtp, N[k]));
+                        off[k] = vtp - N[k] * dp;
+                    }
+                    V3 fixed = pos + interp3(off[0], off[1], off[2], bc);
+                    sh_origin = m34_point(o2w, fixed);
+                    want_shadow = J->shadow_rays;
+                    // read_material_from_textures (:47-63)
+                    float4 dt = sample_index(S, gi.diffuse_image_index, uv.x, uv.y);
+                    float4 mr = sample_index(S, gi.metallic_roughness_image_index, uv.x, uv.y);
+                    V3 base = v3(dt.x, dt.y, dt.z);
+                    float metallic = mr.z, roughness = mr.y;
+                    // calculate_normal (:141-157)
+                    float w2o[12];
+                    m34_inverse(o2w, w2o);
+                    V3 normal;
+                    if (gi.normal_map_image_index < 0) {
+                        normal = normalize(m34_tdir(w2o, nrm));
+                    } else {
+                        float4 mt = sample_index(S, (uint32_t)gi.normal_map_image_index, uv.x, uv.y);
+                        V3 mapn = v3(mt.x * 2.0f - 1.0f, mt.y * 2.0f - 1.0f, mt.z * 2.0f - 1.0f);
+                        V3 dp1 = pb - pa, dp2 = pc - pa;
+                        float du1x = ub.x - ua.x, du1y = ub.y - ua.y, du2x = uc.x - ua.x, du2y = uc.y - ua.y;
+                        V3 dp2perp = cross(dp2, nrm), dp1perp = cross(nrm, dp1);
+                        V3 Tt = dp2perp * du1x + dp1perp * du2x;
+                        V3 Bt = dp2perp * du1y + dp1perp * du2y;
+                        float invmax = 1.0f / sqrtf(fmaxf(dot(Tt, Tt), dot(Bt, Bt)));
+                        V3 tt = Tt * invmax, bb = Bt * invmax;
+                        V3 m = normalize(mapn);
+                        V3 local = (tt * m.x + bb * m.y) + nrm * m.z;
+                        normal = normalize(m34_tdir(w2o, local));
+                    }
+                    float NoL;
+                    V3 comb;
+                    brdf_parts(normal, v3(-d.x, -d.y, -d.z), sun, roughness, base, metallic, 0.5f, NoL, comb);
+                    F.shade_a[p] = make_float4(NoL, comb.x, comb.y, comb.z);
+                    F.shade_b[p] = make_float4(0.1f * base.x, 0.1f * base.y, 0.1f * base.z, 1.0f);
+                } else if (group == RTG_HIT_MIRROR) {
+                    // closest_hit_mirror (closest_hit_mirror.glsl:11-29)
+                    const rtg_model_info mi = S.models[custom];
+                    const rtg_geometry_info gi = reinterpret_cast<const rtg_geometry_info*>(mi.geometry_info_address)[geom];
+                    const uint32_t* idx = reinterpret_cast<const uint32_t*>(gi.index_buffer_address) + 3ull * prim;
+                    V3 na = ld3(mi.normal_buffer_address, idx[0]), nb = ld3(mi.normal_buffer_address, idx[1]),
+                       nc = ld3(mi.normal_buffer_address, idx[2]);
+                    V3 ni = interp3(na, nb, nc, bc);
+                    float w2o[12];
+                    m34_inverse(o2w, w2o);
+                    V3 nn = normalize(m34_tdir(w2o, ni));
+                    float k2 = 2.0f * dot(nn, d);
+                    next_d = d - nn * k2;
+                    next_o = o + d * t;
+                } else if (group == RTG_HIT_PORTAL) {
+                    // closest_hit_portal (lib.rs:300-312)
+                    next_d = d;
+                    next_o = (o + d * t) + v3(0.0f, 5.0f, 0.0f);
+                }
+                bool nonzero = !(next_d.x == 0.0f && next_d.y == 0.0f && next_d.z == 0.0f);
+                want_next = nonzero && (uint32_t)(level + 1) < J->max_traces;
+            }
+        }
+        // compaction: one atomic per wave for each queue
+        uint32_t sbase = wave_alloc(want_shadow, &F.counts[4]);
+        uint32_t nbase = wave_alloc(want_next ? 1u : 0u, next_count);
+        if (want_next) {
+            NQ.o[nbase] = make_float4(next_o.x, next_o.y, next_o.z, 0.01f);
+            NQ.d[nbase] = make_float4(next_d.x, next_d.y, next_d.z, 10000.0f);
+            NQ.path[nbase] = p;
+            n_bounce++;
+        }
+        if (want_shadow) {
+            const SceneView& S = J->scene;
+            for (uint32_t k = 0; k < want_shadow; ++k) {
+                // sample_blue_noise (:99-111) and animated_blue_noise (:116-120)
+                uint32_t o1x = k * 2u * 13u, o1y = k * 2u * 41u;
+                uint32_t o2x = (k * 2u + 1u) * 13u, o2y = (k * 2u + 1u) * 41u;
+                float4 t1 = sample_index(S, bn_index, (float)(px + o1x) / 64.0f, (float)(py + o1y) / 64.0f);
+                float4 t2 = sample_index(S, bn_index, (float)(px + o2x) / 64.0f, (float)(py + o2y) / 64.0f);
+                float anim = (float)(frame % 32u) * 0.618033988749f;
+                float rx = fractf(t1.x + anim), ry = fractf(t2.x + anim);
+                V3 sd = sample_dir_light(rx, ry, sun, sun_radius);
+                uint32_t slot = sbase + k;
+                F.sq.o[slot] = make_float4(sh_origin.x, sh_origin.y, sh_origin.z, 0.001f);
+                F.sq.d[slot] = make_float4(sd.x, sd.y, sd.z, 10000.0f);
+                F.sq.path[slot] = p;
+            }
+            n_shadow += want_shadow;
+        }
+    }
+    if (COUNT) {
+        wave_add_counter(&F.counters[1], n_bounce);
+        wave_add_counter(&F.counters[2], n_shadow);
+    }
+}
+
+// Occlusion of the shadow rays (TerminateOnFirstHit | SkipClosestHit, hit
+// group offset 1, miss shader 1: closest_hit_textured.glsl:159-172);
+// shadow_ray_miss (lib.rs:33-36) makes the ray count as visible.
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion(Frame F, uint32_t* status) {
+    __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
+    const uint32_t n = F.counts[4];
+    const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
+    Stack st;
+    st.lds = s_stack + threadIdx.x;
+    st.stride = kTraceThreads;
+    st.spill = F.spill + (size_t)gtid * kSpillDepth;
+    TravCounters cnt = {0, 0, 0, 0};
+    const uint32_t stride = gridDim.x * kTraceThreads;
+    const uint32_t rounds = (n + stride - 1) / stride;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        uint32_t i = r * stride + gtid;
+        if (i < n) {
+            uint32_t p = F.sq.path[i];
+            const JobDev& J = F.jobs[find_job(F.jobs, F.njobs, p, false)];
+            const TlasView T = tlas_view(J.tlas);
+            float4 a = F.sq.o[i], b = F.sq.d[i];
+            bool heat = (J.flags & RTG_TRACE_HEATMAP) != 0;
+            uint64_t c0 = heat ? __builtin_readcyclecounter() : 0;
+            HitRec h;
+            int res = traverse<true, COUNT>(T, J.scene, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w,
+                                            RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RTG_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER,
+                                            0xffu, 1u, h, st, cnt);
+            if (heat) atomicAdd(&F.cycles[p], (uint32_t)(__builtin_readcyclecounter() - c0));
+            if (res < 0) atomicOr(status, (uint32_t)kStatusStackOverflow);
+            if (res == 0) atomicAdd(&F.vis[p], 1u);
+        }
+    }
+    if (COUNT) flush_counters(F.counters, cnt, kCountersOcclusion);
+}
+
+// linear_to_srgb (lib.rs:84-92)
+__device__ inline float srgb1(float x) {
+    if (x <= 0.0031308f) return x * 12.92f;
+    return 1.055f * pow_own(x, 1.0f / 2.4f) - 0.055f;
+}
+
+__device__ inline float smoothstep_(float e0, float e1, float x) {
+    float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return (t * t) * (3.0f - 2.0f * t);
+}
+
+// heatmap_temperature (shaders/ray-tracing/src/heatmap.rs:5-38)
+__device__ inline V3 heatmap_temperature(float heat) {
+    const float col[10][3] = {{0.0f, 2.0f, 91.0f},     {0.0f, 108.0f, 251.0f}, {0.0f, 221.0f, 221.0f},
+                              {51.0f, 221.0f, 0.0f},   {255.0f, 252.0f, 0.0f}, {255.0f, 180.0f, 0.0f},
+                              {255.0f, 104.0f, 0.0f},  {226.0f, 22.0f, 0.0f},  {191.0f, 0.0f, 83.0f},
+                              {145.0f, 0.0f, 65.0f}};
+    float h = clampf(heat, 0.0f, 1.0f) * 10.0f;
+    int hi = (int)h;
+    int cur = min(hi, 9), prv = max(hi - 1, 0), nxt = min(hi + 1, 9);
+    float hf = floorf(h), hc = ceilf(h);
+    float blur = 0.8f;
+    float wc = smoothstep_(hf - blur, hf + blur, h) * (1.0f - smoothstep_(hc - blur, hc + blur, h));
+    float wp = 1.0f - smoothstep_(hf - blur, hf + blur, h);
+    float wn = smoothstep_(hc - blur, hc + blur, h);
+    float o[3];
+    for (int c = 0; c < 3; ++c) {
+        float r = (wc * (col[cur][c] / 255.0f) + wp * (col[prv][c] / 255.0f)) + wn * (col[nxt][c] / 255.0f);
+        o[c] = clampf(r, 0.0f, 1.0f);
+    }
+    return v3(o[0], o[1], o[2]);
+}
+
+__global__ __launch_bounds__(256) void k_resolve(Frame F, uint64_t total_pixels) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < total_pixels; q += stride) {
+        const JobDev& J = F.jobs[find_job(F.jobs, F.njobs, q, true)];
+        uint32_t pix = (uint32_t)(q - J.pixel_base);
+        V3 acc = v3(0.0f, 0.0f, 0.0f);
+        uint32_t cycles = 0;
+        for (uint32_t s = 0; s < J.spp; ++s) {
+            uint64_t p = J.sample_base + (uint64_t)pix * J.spp + s;
+            float4 b = F.shade_b[p];
+            V3 c;
+            if (b.w != 0.0f) {
+                float4 a = F.shade_a[p];
+                float sf = (float)F.vis[p] / (float)J.shadow_rays;
+                float in_nol = sf * a.x;
+                c = v3(a.y * in_nol + b.x, a.z * in_nol + b.y, a.w * in_nol + b.z);
+            } else {
+                float4 r = F.path_rgb[p];
+                c = v3(r.x, r.y, r.z);
+            }
+            acc = acc + c;
+            cycles += F.cycles[p];
+        }
+        V3 col = J.spp == 1 ? acc : acc * (1.0f / (float)J.spp);
+        if (J.flags & RTG_TRACE_HEATMAP) {
+            // lib.rs:174-186: heatmap_temperature(cycles / 1e6) + colour * 1e-6
+            V3 hm = heatmap_temperature((float)cycles / 1000000.0f);
+            col = hm + col * 0.000001f;
+        }
+        if (J.out_rgb) {
+            J.out_rgb[3ull * pix] = col.x;
+            J.out_rgb[3ull * pix + 1] = col.y;
+            J.out_rgb[3ull * pix + 2] = col.z;
+        }
+        if (J.out_rgba8) {
+            uint32_t r = (uint32_t)(clampf(srgb1(col.x), 0.0f, 1.0f) * 255.0f + 0.5f);
+            uint32_t g = (uint32_t)(clampf(srgb1(col.y), 0.0f, 1.0f) * 255.0f + 0.5f);
+            uint32_t b = (uint32_t)(clampf(srgb1(col.z), 0.0f, 1.0f) * 255.0f + 0.5f);
+            J.out_rgba8[pix] = r | (g << 8) | (b << 16) | (255u << 24);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct FrameArena {
+    DeviceBuffer buf;
+    DeviceBuffer spill;
+    uint64_t paths_cap = 0, shadow_cap = 0;
+    uint32_t jobs_cap = 0;
+    JobDev* h_jobs = nullptr;  // pinned staging
+    uint32_t h_jobs_cap = 0;
+    hipEvent_t staging_free = nullptr;
+    Frame F;
+    JobDev* d_jobs = nullptr;
+    uint32_t* d_counts = nullptr;
+};
+
+void free_frame_arena(rtg_ctx* ctx) {
+    if (!ctx->frame) return;
+    FrameArena* a = ctx->frame;
+    a->buf.release();
+    a->spill.release();
+    if (a->h_jobs) hipHostFree(a->h_jobs);
+    if (a->staging_free) hipEventDestroy(a->staging_free);
+    delete a;
+    ctx->frame = nullptr;
+}
+
+static int trace_grid(rtg_ctx* ctx) {
+    // LDS stack of 32 KB per block: 5 blocks per CU are resident.
+    return ctx->cu_count * 5;
+}
+
+static int ensure_spill(rtg_ctx* ctx, FrameArena* a) {
+    size_t need = (size_t)trace_grid(ctx) * kTraceThreads * kSpillDepth * sizeof(uint2);
+    return a->spill.ensure(need);
+}
+
+static FrameArena* arena(rtg_ctx* ctx) {
+    if (!ctx->frame) ctx->frame = new FrameArena();
+    return ctx->frame;
+}
+
+static int ensure_frame(rtg_ctx* ctx, uint64_t paths, uint64_t shadow, uint32_t njobs) {
+    FrameArena* a = arena(ctx);
+    if (paths > a->paths_cap || shadow > a->shadow_cap || njobs > a->jobs_cap || a->buf.ptr == nullptr) {
+        uint64_t P = std::max<uint64_t>(paths, a->paths_cap);
+        uint64_t S = std::max<uint64_t>(shadow, a->shadow_cap);
+        uint32_t NJ = std::max<uint32_t>(njobs, a->jobs_cap);
+        size_t off = 0;
+        carve_size<float4>(off, P);        // path_rgb
+        carve_size<float4>(off, P);        // shade_a
+        carve_size<float4>(off, P);        // shade_b
+        carve_size<uint32_t>(off, P);      // vis
+        carve_size<uint32_t>(off, P);      // cycles
+        for (int q = 0; q < 2; ++q) {
+            carve_size<float4>(off, P);
+            carve_size<float4>(off, P);
+            carve_size<uint32_t>(off, P);
+        }
+        carve_size<uint4>(off, P);
+        carve_size<float2>(off, P);
+        carve_size<float4>(off, S);
+        carve_size<float4>(off, S);
+        carve_size<uint32_t>(off, S);
+        carve_size<uint32_t>(off, 8);
+        carve_size<JobDev>(off, NJ);
+        if (a->buf.ensure(off + 4096) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
+        Carve c(a->buf.ptr);
+        Frame& F = a->F;
+        F.path_rgb = c.take<float4>(P);
+        F.shade_a = c.take<float4>(P);
+        F.shade_b = c.take<float4>(P);
+        F.vis = c.take<uint32_t>(P);
+        F.cycles = c.take<uint32_t>(P);
+        for (int q = 0; q < 2; ++q) {
+            F.q[q].o = c.take<float4>(P);
+            F.q[q].d = c.take<float4>(P);
+            F.q[q].path = c.take<uint32_t>(P);
+        }
+        F.hit_a = c.take<uint4>(P);
+        F.hit_b = c.take<float2>(P);
+        F.sq.o = c.take<float4>(S);
+        F.sq.d = c.take<float4>(S);
+        F.sq.path = c.take<uint32_t>(S);
+        a->d_counts = c.take<uint32_t>(8);
+        a->d_jobs = c.take<JobDev>(NJ);
+        a->paths_cap = P;
+        a->shadow_cap = S;
+        a->jobs_cap = NJ;
+    }
+    if (ensure_spill(ctx, a) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
+    if (njobs > a->h_jobs_cap) {
+        if (a->staging_free) hipEventSynchronize(a->staging_free);
+        if (a->h_jobs) hipHostFree(a->h_jobs);
+        RTG_HIP(hipHostMalloc(reinterpret_cast<void**>(&a->h_jobs), njobs * sizeof(JobDev)));
+        a->h_jobs_cap = njobs;
+    }
+    if (!a->staging_free) RTG_HIP(hipEventCreateWithFlags(&a->staging_free, hipEventDisableTiming));
+    return RTG_OK;
+}
+
+static int validate_job(rtg_ctx* ctx, const rtg_image_job& j, uint32_t k) {
+    auto it = ctx->structures.find(j.tlas);
+    RTG_REQUIRE(it != ctx->structures.end() && it->second.kind == kKindTlas, RTG_ERROR_NOT_FOUND,
+                "job %u: 0x%llx is not a TLAS of this context", k, (unsigned long long)j.tlas);
+    RTG_REQUIRE(j.width > 0 && j.height > 0, RTG_ERROR_INVALID_ARGUMENT, "job %u: empty image %ux%u", k, j.width,
+                j.height);
+    RTG_REQUIRE(j.opts.spp >= 1 && j.opts.spp <= 64, RTG_ERROR_INVALID_ARGUMENT, "job %u: spp %u (1..64)", k,
+                j.opts.spp);
+    RTG_REQUIRE(j.opts.max_traces >= 1 && j.opts.max_traces <= 4, RTG_ERROR_INVALID_ARGUMENT,
+                "job %u: max_traces %u (1..4)", k, j.opts.max_traces);
+    RTG_REQUIRE(j.opts.shadow_rays >= 1 && j.opts.shadow_rays <= 16, RTG_ERROR_INVALID_ARGUMENT,
+                "job %u: shadow_rays %u (1..16)", k, j.opts.shadow_rays);
+    RTG_REQUIRE(j.scene.model_count == 0 || j.scene.model_info, RTG_ERROR_INVALID_ARGUMENT,
+                "job %u: model_count %u with a null model table", k, j.scene.model_count);
+    RTG_REQUIRE(j.scene.texture_count == 0 || j.scene.textures, RTG_ERROR_INVALID_ARGUMENT,
+                "job %u: texture_count %u with a null texture table", k, j.scene.texture_count);
+    return RTG_OK;
+}
+
+int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStream_t s) {
+    RTG_REQUIRE(njobs > 0 && jobs, RTG_ERROR_INVALID_ARGUMENT, "trace: no jobs");
+    uint64_t paths = 0, pixels = 0, shadow = 0;
+    uint32_t max_traces = 0;
+    for (uint32_t k = 0; k < njobs; ++k) {
+        int st = validate_job(ctx, jobs[k], k);
+        if (st != RTG_OK) return st;
+        uint64_t px = (uint64_t)jobs[k].width * jobs[k].height;
+        paths += px * jobs[k].opts.spp;
+        pixels += px;
+        shadow += px * jobs[k].opts.spp * jobs[k].opts.shadow_rays;
+        max_traces = std::max(max_traces, jobs[k].opts.max_traces);
+    }
+    RTG_REQUIRE(paths < (1ull << 31) && shadow < (1ull << 32), RTG_ERROR_LIMIT,
+                "trace: %llu paths / %llu shadow rays exceed one launch", (unsigned long long)paths,
+                (unsigned long long)shadow);
+    RTG_HIP(hipSetDevice(ctx->device));
+    int st = ensure_frame(ctx, paths, shadow, njobs);
+    if (st != RTG_OK) return st;
+    FrameArena* a = ctx->frame;
+    // job table through pinned staging; wait until the previous upload left it
+    RTG_HIP(hipEventSynchronize(a->staging_free));
+    uint64_t sb = 0, pb = 0;
+    for (uint32_t k = 0; k < njobs; ++k) {
+        const rtg_image_job& j = jobs[k];
+        JobDev& d = a->h_jobs[k];
+        memset(&d, 0, sizeof(d));
+        d.tlas = reinterpret_cast<const uint8_t*>(j.tlas);
+        d.scene.models = reinterpret_cast<const rtg_model_info*>(j.scene.model_info);
+        d.scene.textures = reinterpret_cast<const rtg_texture*>(j.scene.textures);
+        d.scene.srgb_lut = ctx->d_srgb_lut;
+        d.scene.model_count = j.scene.model_count;
+        d.scene.texture_count = j.scene.texture_count;
+        memcpy(d.view_inverse, j.uniforms.view_inverse, sizeof(d.view_inverse));
+        memcpy(d.proj_inverse, j.uniforms.proj_inverse, sizeof(d.proj_inverse));
+        memcpy(d.sun, j.uniforms.sun_dir, sizeof(d.sun));
+        d.sun_radius = j.uniforms.sun_radius;
+        d.bn_index = j.uniforms.blue_noise_texture_index;
+        d.frame_index = j.uniforms.frame_index;
+        d.width = j.width;
+        d.height = j.height;
+        d.spp = j.opts.spp;
+        d.shadow_rays = j.opts.shadow_rays;
+        d.max_traces = j.opts.max_traces;
+        d.flags = j.opts.flags | (j.uniforms.show_heatmap ? RTG_TRACE_HEATMAP : 0u);
+        d.sample_base = sb;
+        d.pixel_base = pb;
+        d.out_rgba8 = j.out_rgba8;
+        d.out_rgb = j.out_rgb;
+        d.out_hits = j.out_primary_hits;
+        uint64_t px = (uint64_t)j.width * j.height;
+        sb += px * j.opts.spp;
+        pb += px;
+    }
+    RTG_HIP(hipMemcpyAsync(a->d_jobs, a->h_jobs, njobs * sizeof(JobDev), hipMemcpyHostToDevice, s));
+    RTG_HIP(hipEventRecord(a->staging_free, s));
+    Frame F = a->F;
+    F.jobs = a->d_jobs;
+    F.njobs = njobs;
+    F.paths = paths;
+    F.counts = a->d_counts;
+    F.counters = ctx->d_counters;
+    F.spill = reinterpret_cast<uint2*>(a->spill.ptr);
+    const bool cnt = ctx->counters_enabled;
+    const unsigned G = (unsigned)trace_grid(ctx);
+    const unsigned Gsmall = (unsigned)std::min<uint64_t>((paths + 255) / 256, (uint64_t)ctx->cu_count * 16);
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, a->d_counts, ctx->d_counters, cnt);
+    {
+        LaunchTimer t(ctx, RTG_KERNEL_RAYGEN, s);
+        hipLaunchKernelGGL(k_raygen, dim3(Gsmall), dim3(256), 0, s, F, cnt);
+    }
+    for (uint32_t level = 0; level < max_traces; ++level) {
+        const uint32_t* cp = level == 0 ? nullptr : a->d_counts + level;
+        uint32_t fixed = level == 0 ? (uint32_t)paths : 0u;
+        {
+            LaunchTimer t(ctx, RTG_KERNEL_TRACE_CLOSEST, s);
+            if (cnt)
+                hipLaunchKernelGGL(k_trace_closest<true>, dim3(G), dim3(kTraceThreads), 0, s, F, (int)level, cp, fixed,
+                                   ctx->d_status);
+            else
+                hipLaunchKernelGGL(k_trace_closest<false>, dim3(G), dim3(kTraceThreads), 0, s, F, (int)level, cp,
+                                   fixed, ctx->d_status);
+        }
+        {
+            LaunchTimer t(ctx, RTG_KERNEL_SHADE, s);
+            if (cnt)
+                hipLaunchKernelGGL(k_shade<true>, dim3(Gsmall), dim3(256), 0, s, F, (int)level, cp, fixed);
+            else
+                hipLaunchKernelGGL(k_shade<false>, dim3(Gsmall), dim3(256), 0, s, F, (int)level, cp, fixed);
+        }
+    }
+    {
+        LaunchTimer t(ctx, RTG_KERNEL_TRACE_OCCLUSION, s);
+        if (cnt)
+            hipLaunchKernelGGL(k_trace_occlusion<true>, dim3(G), dim3(kTraceThreads), 0, s, F, ctx->d_status);
+        else
+            hipLaunchKernelGGL(k_trace_occlusion<false>, dim3(G), dim3(kTraceThreads), 0, s, F, ctx->d_status);
+    }
+    {
+        LaunchTimer t(ctx, RTG_KERNEL_RESOLVE, s);
+        unsigned Gr = (unsigned)std::min<uint64_t>((pixels + 255) / 256, (uint64_t)ctx->cu_count * 16);
+        hipLaunchKernelGGL(k_resolve, dim3(Gr), dim3(256), 0, s, F, pixels);
+    }
+    RTG_HIP(hipGetLastError());
+    return RTG_OK;
+}
+
+}  // namespace rtg
+
+using namespace rtg;
+
+extern "C" int rtg_trace_image(rtg_ctx* ctx, const rtg_image_job* job, void* stream) {
+    clear_error();
+    RTG_REQUIRE(ctx && job, RTG_ERROR_INVALID_ARGUMENT, "rtg_trace_image: null argument");
+    return trace_jobs(ctx, job, 1, reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int rtg_trace_batch(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t n, void* stream) {
+    clear_error();
+    RTG_REQUIRE(ctx && (jobs || n == 0), RTG_ERROR_INVALID_ARGUMENT, "rtg_trace_batch: null argument");
+    if (n == 0) return RTG_OK;
+    return trace_jobs(ctx, jobs, n, reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int rtg_trace_rays(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindings* scene, const rtg_ray* rays,
+                              uint32_t count, uint32_t mode, uint32_t ray_flags, uint32_t cull_mask,
+                              uint32_t sbt_offset, rtg_hit* hits, uint32_t* occluded, void* stream) {
+    clear_error();
+    RTG_REQUIRE(ctx, RTG_ERROR_INVALID_ARGUMENT, "rtg_trace_rays: null context");
+    auto it = ctx->structures.find(tlas);
+    RTG_REQUIRE(it != ctx->structures.end() && it->second.kind == kKindTlas, RTG_ERROR_NOT_FOUND,
+                "rtg_trace_rays: 0x%llx is not a TLAS of this context", (unsigned long long)tlas);
+    RTG_REQUIRE(mode <= RTG_QUERY_OCCLUSION, RTG_ERROR_INVALID_ARGUMENT, "rtg_trace_rays: mode %u", mode);
+    RTG_REQUIRE(count == 0 || rays, RTG_ERROR_INVALID_ARGUMENT, "rtg_trace_rays: null rays");
+    RTG_REQUIRE(mode != RTG_QUERY_CLOSEST || count == 0 || hits, RTG_ERROR_INVALID_ARGUMENT,
+                "rtg_trace_rays: null hits");
+    RTG_REQUIRE(mode != RTG_QUERY_OCCLUSION || count == 0 || occluded, RTG_ERROR_INVALID_ARGUMENT,
+                "rtg_trace_rays: null occluded");
+    const uint32_t allowed = RTG_RAY_FLAG_OPAQUE | RTG_RAY_FLAG_NO_OPAQUE | RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT |
+                             RTG_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER;
+    RTG_REQUIRE((ray_flags & ~allowed) == 0, RTG_ERROR_INVALID_ARGUMENT, "rtg_trace_rays: unsupported ray flags 0x%x",
+                ray_flags & ~allowed);
+    if (count == 0) return RTG_OK;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    RTG_HIP(hipSetDevice(ctx->device));
+    FrameArena* a = arena(ctx);
+    if (ensure_spill(ctx, a) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
+    SceneView S;
+    S.models = scene ? reinterpret_cast<const rtg_model_info*>(scene->model_info) : nullptr;
+    S.textures = scene ? reinterpret_cast<const rtg_texture*>(scene->textures) : nullptr;
+    S.srgb_lut = ctx->d_srgb_lut;
+    S.model_count = scene ? scene->model_count : 0;
+    S.texture_count = scene ? scene->texture_count : 0;
+    const bool cnt = ctx->counters_enabled;
+    const unsigned G = (unsigned)std::min<uint64_t>((count + kTraceThreads - 1) / kTraceThreads, trace_grid(ctx));
+    uint8_t* tl = reinterpret_cast<uint8_t*>(tlas);
+    uint2* spill = reinterpret_cast<uint2*>(a->spill.ptr);
+    // the primary ray count of a ray query is the query size
+    if (cnt) hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, ctx->d_counters, (uint64_t)count);
+    LaunchTimer t(ctx, mode == RTG_QUERY_CLOSEST ? RTG_KERNEL_TRACE_CLOSEST : RTG_KERNEL_TRACE_OCCLUSION, s);
+    cull_mask &= 0xffu;
+#define RTG_LAUNCH_RAYS(OCC, CNT)                                                                                   \
+    hipLaunchKernelGGL((k_trace_rays<OCC, CNT>), dim3(G), dim3(kTraceThreads), 0, s, tl, S, rays, count, ray_flags, \
+                       cull_mask, sbt_offset, hits, occluded, spill, ctx->d_counters, ctx->d_status)
+    if (mode == RTG_QUERY_CLOSEST) {
+        if (cnt) RTG_LAUNCH_RAYS(false, true);
+        else RTG_LAUNCH_RAYS(false, false);
+    } else {
+        if (cnt) RTG_LAUNCH_RAYS(true, true);
+        else RTG_LAUNCH_RAYS(true, false);
+    }
+#undef RTG_LAUNCH_RAYS
+    RTG_HIP(hipGetLastError());
+    return RTG_OK;
+}

@@ -1,0 +1,35 @@
+import os
+import subprocess
+import sys
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(REPO, "ray-tracing-gallery_amd")
+sys.path.insert(0, PKG)
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+    orc = os.path.join(REPO, "oracle", "_build", "liborc.so")
+    if not os.path.exists(orc):
+        subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True, capture_output=True)
+
+
+def gpu_available():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    from rtg.lib import Context
+    if not gpu_available():
+        pytest.skip("no GPU")
+    c = Context(0)
+    yield c
+    c.close()

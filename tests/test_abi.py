@@ -1,0 +1,67 @@
+"""The C ABI: struct layouts of include/rtg.h and the exported symbols of
+librtg.so (no compute calls; runs without a GPU)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import PKG, REPO
+from rtg import abi, lib
+
+HEADER = os.path.join(REPO, "include", "rtg.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rtg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_binding_list():
+    assert declared_functions() == sorted(lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(lib.LIB_PATH):
+        subprocess.run(["make", "-C", os.path.join(PKG, "csrc")], check=True, capture_output=True)
+    L = C.CDLL(lib.LIB_PATH)
+    missing = [f for f in declared_functions() if not hasattr(L, f)]
+    assert not missing, missing
+    assert lib.load().rtg_abi_version() == 1
+
+
+def test_struct_layouts_match_the_reference_abi():
+    # vk::AccelerationStructureInstanceKHR (src/gpu_structs.rs:17-25)
+    assert C.sizeof(abi.Instance) == 64
+    assert abi.Instance.custom_index_and_mask.offset == 48
+    assert abi.Instance.sbt_offset_and_flags.offset == 52
+    assert abi.Instance.acceleration_structure_reference.offset == 56
+    # Uniforms (shared-structs/src/lib.rs:7-19): Vec3A sun_dir at 128, 176 B
+    assert abi.Uniforms.sun_dir.offset == 128
+    assert abi.Uniforms.sun_radius.offset == 144
+    assert abi.Uniforms.blue_noise_texture_index.offset == 148
+    assert abi.Uniforms.ggx_lut_texture_index.offset == 152
+    assert abi.Uniforms.frame_index.offset == 156
+    assert abi.Uniforms.show_heatmap.offset == 160
+    assert C.sizeof(abi.Uniforms) == 176
+    # ModelInfo / GeometryInfo (shared-structs/src/lib.rs:21-47)
+    assert C.sizeof(abi.ModelInfo) == 32
+    assert C.sizeof(abi.GeometryInfo) == 24
+    assert abi.GeometryInfo.normal_map_image_index.offset == 16
+
+
+def test_packed24_8():
+    # vk::Packed24_8::new(low24, high8) = low24 | high8 << 24
+    assert abi.pack24_8(5, 0xFF) == 0xFF000005
+    assert abi.pack24_8(0xFFFFFF, 0) == 0x00FFFFFF
+    assert abi.pack24_8(0x1000001, 1) == 0x01000001  # low 24 bits only
+
+
+def test_context_without_gpu_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(lib.RtgError):
+        lib.Context(0)

@@ -1,0 +1,170 @@
+"""HIP path vs the CPU oracle: acceleration-structure bytes, hit records,
+occlusion and shaded colour. Integer and index results are compared bit for
+bit; shaded linear RGB within 1e-4 absolute."""
+import numpy as np
+import pytest
+import torch
+
+import oracle_host as oh
+from rtg import abi, scenes
+from rtg.device import DeviceScene, hits_to_numpy, rays_to_device
+
+pytestmark = pytest.mark.gpu
+
+RGB_TOL = 1e-4
+
+
+def tlas_bytes_masked(blob, n):
+    """TLAS blob bytes with the BLAS address fields zeroed (they are device
+    addresses on the GPU and host pointers in the oracle)."""
+    b = np.frombuffer(blob, dtype=np.uint8).copy()
+    hdr = b[:128].view(np.uint64)
+    prims_off, extra_off = int(hdr[9]), int(hdr[10])
+    recs = b[prims_off:prims_off + 64 * n].reshape(n, 64)
+    recs[:, 48:56] = 0
+    src = b[extra_off:extra_off + 64 * n].reshape(n, 64)
+    src[:, 56:64] = 0
+    return b
+
+
+@pytest.fixture(scope="module")
+def small(ctx):
+    s = scenes.small_scene(width=128, height=96)
+    return s, DeviceScene(ctx, s), oh.HostScene(s)
+
+
+def test_blas_bytes_match_oracle(ctx, small):
+    s, ds, hs = small
+    for k, m in enumerate(s.meshes):
+        g = ds.as_bytes(ds.blas[k])
+        o = np.frombuffer(hs.blas[k].bytes(), dtype=np.uint8)
+        assert g.shape == o.shape, (m.name, g.shape, o.shape)
+        assert np.array_equal(g, o), m.name
+
+
+@pytest.mark.parametrize("mesh", ["torus", "icosphere", "soup", "one", "empty", "multi"])
+def test_blas_shapes(ctx, mesh):
+    if mesh == "torus":
+        m = scenes.torus(48, 24)
+    elif mesh == "icosphere":
+        m = scenes.icosphere(3)
+    elif mesh == "soup":
+        m = scenes.soup(5000, 0.05, rng=scenes.Rng(5))
+    elif mesh == "one":
+        m = scenes.Mesh("one", [[0, 0, 0], [1, 0, 0], [0, 1, 0]], [[0, 0, 1]] * 3, [[0, 0]] * 3,
+                        [(np.array([[0, 1, 2]]), True)])
+    elif mesh == "empty":
+        m = scenes.Mesh("empty", [[0, 0, 0]], [[0, 0, 1]], [[0, 0]], [(np.zeros((0, 3)), True)])
+    else:
+        t = scenes.torus(12, 6)
+        ix = t.geometries[0][0]
+        m = scenes.Mesh("multi", t.positions, t.normals, t.uvs, [(ix[:50], True), (ix[50:], False)])
+    s = scenes.Scene("blas")
+    s.meshes.append(m)
+    s.models.append(scenes.Model(0, [(0, 0, -1)]))
+    s.instances.append(scenes.InstanceData(scenes.affine((0, 0, 0)), 0, 0))
+    ds = DeviceScene(ctx, s)
+    hs = oh.HostScene(s)
+    g = ds.as_bytes(ds.blas[0])
+    o = np.frombuffer(hs.blas[0].bytes(), dtype=np.uint8)
+    assert np.array_equal(g, o)
+    ds.destroy()
+
+
+def test_tlas_bytes_and_refit_match_oracle(ctx, small):
+    s, ds, hs = small
+    n = len(s.instances)
+    g = tlas_bytes_masked(ds.as_bytes(ds.tlas), n)
+    o = tlas_bytes_masked(hs.tlas.bytes(), n)
+    assert np.array_equal(g, o)
+    # refit a clone with new rotations (src == dst UPDATE semantics)
+    rng = scenes.Rng(99)
+    xf = [i.transform.copy() for i in s.instances]
+    for k in range(1, n):
+        xf[k] = scenes.affine(xf[k][:, 3], scenes.rotation_from_uniform(*rng.uniform(3)), 1.3)
+    clone = ds.clone_tlas()
+    buf = ds.instance_buffer(xf)
+    ds.refit(buf, tlas=clone)
+    torch.cuda.synchronize()
+    hs2 = oh.HostScene(s)
+    hs2.refit(xf)
+    g2 = tlas_bytes_masked(ds.as_bytes(clone), n)
+    o2 = tlas_bytes_masked(hs2.tlas.bytes(), n)
+    assert np.array_equal(g2, o2)
+
+
+def _compare_rays(ds, hs, rays, mode, **kw):
+    d = rays_to_device(rays)
+    out = ds.trace_rays(d, len(rays), mode=mode, **kw)
+    torch.cuda.synchronize()
+    ref, _ = hs.trace_rays(rays, mode=mode, **kw)
+    if mode == abi.RTG_QUERY_CLOSEST:
+        got = hits_to_numpy(out)
+        assert np.array_equal(got.view(np.uint32).reshape(-1, 6), ref.view(np.uint32).reshape(-1, 6))
+    else:
+        got = out.cpu().numpy().astype(np.uint32)
+        assert np.array_equal(got, ref)
+    return ref
+
+
+@pytest.mark.parametrize("mode", [abi.RTG_QUERY_CLOSEST, abi.RTG_QUERY_OCCLUSION])
+def test_trace_rays_small(ctx, small, mode):
+    s, ds, hs = small
+    rays = scenes.incoherent_rays(50000, (-10, -1, -10), (10, 4, 10), seed=11)
+    ref = _compare_rays(ds, hs, rays, mode)
+    if mode == abi.RTG_QUERY_CLOSEST:
+        assert (ref["instance"] != abi.RTG_NO_HIT).mean() > 0.2
+
+
+def test_trace_rays_counters_match_oracle(ctx, small):
+    s, ds, hs = small
+    rays = scenes.incoherent_rays(20000, (-10, -1, -10), (10, 4, 10), seed=12)
+    ctx.set_counters(True)
+    try:
+        d = rays_to_device(rays)
+        ds.trace_rays(d, len(rays))
+        c = ctx.counters()
+    finally:
+        ctx.set_counters(False)
+    _, oc = hs.trace_rays(rays)
+    for f in ("node_visits", "triangle_tests", "instance_visits"):
+        assert getattr(c, f) == getattr(oc, f), f
+
+
+def _compare_image(ds, hs, frame=None, opts=None, rgb_tol=RGB_TOL):
+    outs = ds.trace_image(frame_index=frame, opts=opts)
+    torch.cuda.synchronize()
+    rgba, rgb, ph = outs
+    o_rgba, o_rgb, o_hits, _ = hs.trace_image(frame_index=frame, opts=opts)
+    got_hits = hits_to_numpy(ph)
+    assert np.array_equal(got_hits["instance"], o_hits["instance"])
+    assert np.array_equal(got_hits["primitive"], o_hits["primitive"])
+    assert np.array_equal(got_hits["geometry"], o_hits["geometry"])
+    assert np.array_equal(got_hits.view(np.uint32), o_hits.view(np.uint32))
+    g_rgb = rgb.cpu().numpy().reshape(-1, 3)
+    err = np.abs(g_rgb - o_rgb).max()
+    assert err <= rgb_tol, err
+    g_rgba = rgba.cpu().numpy().view(np.uint32)
+    assert np.array_equal(g_rgba, o_rgba)
+    return o_rgb
+
+
+def test_trace_image_small(ctx, small):
+    s, ds, hs = small
+    _compare_image(ds, hs)
+
+
+def test_trace_image_large_reduced(ctx):
+    s = scenes.large_scene(width=192, height=108)
+    ds = DeviceScene(ctx, s)
+    hs = oh.HostScene(s)
+    _compare_image(ds, hs)
+    ds.destroy()
+
+
+def test_trace_image_multisample(ctx, small):
+    s, ds, hs = small
+    o = s.trace_opts()
+    o.spp = 3
+    o.shadow_rays = 2
+    _compare_image(ds, hs, frame=5, opts=o)
