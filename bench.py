@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity check")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first run one elementwise kernel that reads and writes BYTES (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build measures one GPU: the trace step has no cross-GPU exchange; "
@@ -68,6 +70,16 @@ def main():
     from rtg import abi, scenes
     from rtg.device import DeviceScene, hits_to_numpy
     from rtg.lib import Context
+
+    if args.copy_calibration:
+        n = args.copy_calibration // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.neg(src, out=dst)  # reads n*4 bytes, writes n*4 bytes
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"copy calibration: {n * 4} bytes read and written")
 
     t_setup = time.perf_counter()
     scene = scenes.large_scene(width=args.width, height=args.height)

@@ -1,0 +1,65 @@
+"""Writes the golden fixtures of the trace step with the CPU oracle.
+
+Each fixture stores every input (meshes, textures, instances, camera,
+options) and the oracle's outputs, so the checks never depend on
+regenerating a scene on another machine:
+  small_frame.npz    small tori scene, 128x96, 1 hard shadow ray
+  large_mini.npz     reduced animated scene (tori, icospheres, soups, alpha
+                     fences, mirrors), 96x54, 4 soft-shadow rays, frame 3
+  small_spp.npz      small scene, 64x48, 3 samples, 2 shadow rays, frame 7
+  rays_small.npz     incoherent closest-hit and occlusion queries
+Run: python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(HERE)), "ray-tracing-gallery_amd"))
+
+import oracle_host as oh  # noqa: E402
+from rtg import abi, scenes  # noqa: E402
+
+
+def frame_fixture(name, scene, frame=None):
+    hs = oh.HostScene(scene)
+    rgba, rgb, hits, cnt = hs.trace_image(frame_index=frame)
+    arrays = scene.to_arrays()
+    arrays["out_rgba8"] = rgba
+    arrays["out_rgb"] = rgb
+    arrays["out_hits"] = hits.view(np.uint32).reshape(-1, 6)
+    arrays["out_counters"] = np.array([getattr(cnt, f) for f, _ in abi.TraceCounters._fields_], np.uint64)
+    arrays["frame"] = np.array([scene.frame_index if frame is None else frame], np.int64)
+    np.savez_compressed(os.path.join(HERE, name), **arrays)
+    print(name, scene.width, scene.height, "hit fraction", float((hits["instance"] != abi.RTG_NO_HIT).mean()))
+
+
+def main():
+    frame_fixture("small_frame.npz", scenes.small_scene(width=128, height=96))
+    mini = scenes.large_scene(width=96, height=54, n_objects=40, n_fences=10, n_mirrors=6, n_soup_meshes=2,
+                              soup_tris=200)
+    mini.set_frame(3)
+    frame_fixture("large_mini.npz", mini, frame=3)
+    s = scenes.small_scene(width=64, height=48)
+    s.spp = 3
+    s.shadow_rays = 2
+    s.frame_index = 7
+    frame_fixture("small_spp.npz", s)
+    # ray queries
+    s = scenes.small_scene(width=16, height=16)
+    hs = oh.HostScene(s)
+    rays = scenes.incoherent_rays(3000, (-10, -1, -10), (10, 4, 10), seed=31)
+    hits, _ = hs.trace_rays(rays)
+    occ, _ = hs.trace_rays(rays, mode=abi.RTG_QUERY_OCCLUSION)
+    arrays = s.to_arrays()
+    arrays["rays"] = rays.view(np.uint32).reshape(-1, 8)
+    arrays["out_hits"] = hits.view(np.uint32).reshape(-1, 6)
+    arrays["out_occluded"] = occ
+    np.savez_compressed(os.path.join(HERE, "rays_small.npz"), **arrays)
+    print("rays_small.npz", len(rays))
+
+
+if __name__ == "__main__":
+    main()
