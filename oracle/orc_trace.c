@@ -146,9 +146,11 @@ static void oray_setup(oray* r, v3 o, v3 d) {
     r->kx = kx;
     r->ky = ky;
     r->kz = kz;
-    r->sx = dv[kx] / dv[kz];
-    r->sy = dv[ky] / dv[kz];
-    r->sz = 1.0f / dv[kz];
+    /* sz = 1/d[kz] (the inverse already computed), sx = d[kx]*sz, sy = d[ky]*sz */
+    float iv[3] = {r->inv.x, r->inv.y, r->inv.z};
+    r->sz = iv[kz];
+    r->sx = dv[kx] * r->sz;
+    r->sy = dv[ky] * r->sz;
 }
 
 /* Returns 1 on a hit with tmin <= t <= tmax; (u, v) weight vertices 1, 2. */

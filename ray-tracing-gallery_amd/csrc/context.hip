@@ -2,6 +2,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "rtg_device.hpp"
@@ -99,6 +100,18 @@ extern "C" int rtg_create(rtg_ctx** out, int device) {
     rtg_ctx* ctx = new rtg_ctx();
     ctx->device = device;
     ctx->cu_count = prop.multiProcessorCount;
+    auto env_int = [](const char* name, int def, int lo, int hi) {
+        const char* v = getenv(name);
+        if (!v || !*v) return def;
+        int x = atoi(v);
+        return x < lo ? lo : (x > hi ? hi : x);
+    };
+    ctx->tune.refill_closest = env_int("RTG_REFILL_CLOSEST", ctx->tune.refill_closest, 0, 64);
+    ctx->tune.refill_occlusion = env_int("RTG_REFILL_OCCLUSION", ctx->tune.refill_occlusion, 0, 64);
+    ctx->tune.path_order = env_int("RTG_PATH_ORDER", ctx->tune.path_order, 0, 1);
+    ctx->tune.blocks_per_cu = env_int("RTG_TRACE_BLOCKS_PER_CU", ctx->tune.blocks_per_cu, 1, 8);
+    // RTG_FEED_CHUNK counts 64-ray batches taken per atomic
+    ctx->tune.feed_chunk = 64 * env_int("RTG_FEED_CHUNK", ctx->tune.feed_chunk / 64, 1, 16);
     // IEC 61966-2-1 decode of the 256 sRGB codes, in double, rounded once.
     float lut[256];
     for (int i = 0; i < 256; ++i) {

@@ -162,6 +162,30 @@ __device__ inline void mat4_mul(const float* m, const float* v, float* out) {
 __device__ inline float u2f(uint32_t u) { return __uint_as_float(u); }
 __device__ inline uint32_t f2u(float f) { return __float_as_uint(f); }
 
+// Loads through the global address space. Pointers read from memory (BLAS
+// addresses, table entries) are generic to the compiler, and generic (FLAT)
+// loads share a wait counter with the LDS traversal stack; these casts make
+// them global loads.
+#define RTG_GLOBAL __attribute__((address_space(1)))
+typedef float rtg_f4v __attribute__((ext_vector_type(4)));
+typedef uint32_t rtg_u4v __attribute__((ext_vector_type(4)));
+typedef uint32_t rtg_u2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 gld_f4(const void* p) {
+    rtg_f4v v = *(const RTG_GLOBAL rtg_f4v*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 gld_u4(const void* p) {
+    rtg_u4v v = *(const RTG_GLOBAL rtg_u4v*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 gld_u2(const void* p) {
+    rtg_u2v v = *(const RTG_GLOBAL rtg_u2v*)p;
+    return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ uint32_t gld_u32(const void* p) { return *(const RTG_GLOBAL uint32_t*)p; }
+__device__ __forceinline__ uint64_t gld_u64(const void* p) { return *(const RTG_GLOBAL uint64_t*)p; }
+__device__ __forceinline__ float gld_f32(const void* p) { return *(const RTG_GLOBAL float*)p; }
+
 constexpr float kPi = 3.14159265358979323846f;
 
 // sin and cos: Cody-Waite reduction by pi/2 (parts of 12, 12 and 24 bits),
@@ -262,14 +286,14 @@ __device__ inline int wrap_coord(float f, uint32_t size) {
 
 __device__ inline float4 fetch_texel(const rtg_texture& t, const float* lut, int x, int y) {
     uint64_t idx = (uint64_t)y * t.width + (uint64_t)x;
-    if (t.format == RTG_FORMAT_RGBA32_FLOAT) return reinterpret_cast<const float4*>(t.data)[idx];
-    uint32_t p = reinterpret_cast<const uint32_t*>(t.data)[idx];
+    if (t.format == RTG_FORMAT_RGBA32_FLOAT) return gld_f4(reinterpret_cast<const float4*>(t.data) + idx);
+    uint32_t p = gld_u32(reinterpret_cast<const uint32_t*>(t.data) + idx);
     uint32_t r = p & 0xffu, g = (p >> 8) & 0xffu, b = (p >> 16) & 0xffu, a = p >> 24;
     float4 o;
     if (t.format == RTG_FORMAT_RGBA8_SRGB) {
-        o.x = lut[r];
-        o.y = lut[g];
-        o.z = lut[b];
+        o.x = gld_f32(lut + r);
+        o.y = gld_f32(lut + g);
+        o.z = gld_f32(lut + b);
     } else {
         o.x = (float)r / 255.0f;
         o.y = (float)g / 255.0f;
@@ -306,7 +330,14 @@ __device__ inline float4 sample_tex(const rtg_texture& t, const float* lut, floa
 
 __device__ inline float4 sample_index(const SceneView& S, uint32_t index, float u, float v) {
     if (index >= S.texture_count) return make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-    rtg_texture t = S.textures[index];
+    const rtg_texture* tp = S.textures + index;
+    rtg_texture t;
+    t.data = gld_u64(&tp->data);
+    uint4 rest = gld_u4(&tp->width);
+    t.width = rest.x;
+    t.height = rest.y;
+    t.format = rest.z;
+    t.filter = rest.w;
     return sample_tex(t, S.srgb_lut, u, v);
 }
 
@@ -321,10 +352,17 @@ struct RayCtx {
 
 __device__ inline float sel3(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
-__device__ inline void ray_setup(RayCtx& r, V3 o, V3 d) {
+__device__ inline void ray_setup_box(RayCtx& r, V3 o, V3 d) {
     r.o = o;
     r.d = d;
     r.inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+}
+
+// Box and shear setup of the watertight test: kz = the dominant axis,
+// (kx, ky) the others (swapped if d[kz] < 0), sz = 1/d[kz],
+// sx = d[kx]*sz, sy = d[ky]*sz.
+__device__ inline void ray_setup(RayCtx& r, V3 o, V3 d) {
+    ray_setup_box(r, o, d);
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
     int kx = kz + 1 == 3 ? 0 : kz + 1;
@@ -338,9 +376,9 @@ __device__ inline void ray_setup(RayCtx& r, V3 o, V3 d) {
     r.kx = kx;
     r.ky = ky;
     r.kz = kz;
-    r.sx = sel3(d, kx) / dz;
-    r.sy = sel3(d, ky) / dz;
-    r.sz = 1.0f / dz;
+    r.sz = sel3(r.inv, kz);
+    r.sx = sel3(d, kx) * r.sz;
+    r.sy = sel3(d, ky) * r.sz;
 }
 
 // Watertight ray/triangle test (Woop, Benthin, Wald 2013, JCGT 2(1)).
@@ -428,9 +466,9 @@ __device__ inline TlasView tlas_view(const uint8_t* base) {
     const AsHeader* h = reinterpret_cast<const AsHeader*>(base);
     TlasView v;
     v.base = base;
-    v.nodes = reinterpret_cast<const Node*>(base + h->nodes_off);
-    v.recs = reinterpret_cast<const InstRec*>(base + h->prims_off);
-    v.src = reinterpret_cast<const rtg_instance*>(base + h->extra_off);
+    v.nodes = reinterpret_cast<const Node*>(base + gld_u64(&h->nodes_off));
+    v.recs = reinterpret_cast<const InstRec*>(base + gld_u64(&h->prims_off));
+    v.src = reinterpret_cast<const rtg_instance*>(base + gld_u64(&h->extra_off));
     return v;
 }
 
@@ -453,17 +491,19 @@ __device__ inline bool is_opaque(uint32_t geom_word, uint32_t inst_flags, uint32
 __device__ inline bool alpha_clip_keep(const SceneView& S, const rtg_instance* src, uint32_t inst_index, uint32_t geom,
                                        uint32_t prim, float u, float v) {
     if (S.models == nullptr) return true;
-    uint32_t custom = src[inst_index].custom_index_and_mask & 0xffffffu;
+    uint32_t custom = gld_u32(&src[inst_index].custom_index_and_mask) & 0xffffffu;
     if (custom >= S.model_count) return true;
-    const rtg_model_info mi = S.models[custom];
-    const rtg_geometry_info* gi = reinterpret_cast<const rtg_geometry_info*>(mi.geometry_info_address) + geom;
-    const uint32_t* idx = reinterpret_cast<const uint32_t*>(gi->index_buffer_address) + 3ull * prim;
-    const float* uvs = reinterpret_cast<const float*>(mi.uv_buffer_address);
+    const rtg_model_info* mi = S.models + custom;
+    const rtg_geometry_info* gi =
+        reinterpret_cast<const rtg_geometry_info*>(gld_u64(&mi->geometry_info_address)) + geom;
+    const uint32_t* idx = reinterpret_cast<const uint32_t*>(gld_u64(&gi->index_buffer_address)) + 3ull * prim;
+    const float* uvs = reinterpret_cast<const float*>(gld_u64(&mi->uv_buffer_address));
     V3 bc = v3((1.0f - u) - v, u, v);
-    uint32_t i0 = idx[0], i1 = idx[1], i2 = idx[2];
-    V2 a{uvs[2 * i0], uvs[2 * i0 + 1]}, b{uvs[2 * i1], uvs[2 * i1 + 1]}, c{uvs[2 * i2], uvs[2 * i2 + 1]};
+    uint32_t i0 = gld_u32(idx), i1 = gld_u32(idx + 1), i2 = gld_u32(idx + 2);
+    V2 a{gld_f32(uvs + 2 * i0), gld_f32(uvs + 2 * i0 + 1)}, b{gld_f32(uvs + 2 * i1), gld_f32(uvs + 2 * i1 + 1)},
+        c{gld_f32(uvs + 2 * i2), gld_f32(uvs + 2 * i2 + 1)};
     V2 uv = interp2(a, b, c, bc);
-    float4 tx = sample_index(S, gi->diffuse_image_index, uv.x, uv.y);
+    float4 tx = sample_index(S, gld_u32(&gi->diffuse_image_index), uv.x, uv.y);
     return !(tx.w < 0.5f);
 }
 
@@ -481,121 +521,68 @@ struct Stack {
     int sp;
     __device__ inline void push(uint32_t ref, float tn) {
         uint2 e = make_uint2(ref, f2u(tn));
-        if (sp < kLdsDepth) lds[sp * stride] = e;
-        else spill[sp - kLdsDepth] = e;
+        if (sp < kLdsDepth) {
+            lds[sp * stride] = e;
+        } else {
+            rtg_u2v v = {e.x, e.y};
+            *(RTG_GLOBAL rtg_u2v*)(spill + (sp - kLdsDepth)) = v;
+        }
         ++sp;
     }
     __device__ inline uint2 pop() {
         --sp;
-        return sp < kLdsDepth ? lds[sp * stride] : spill[sp - kLdsDepth];
+        return sp < kLdsDepth ? lds[sp * stride] : gld_u2(spill + (sp - kLdsDepth));
     }
 };
 
-// Returns 1 for an accepted hit, 0 for none, -1 if the stack overflowed.
+// Final states of a traversal step.
+constexpr int kRayActive = 0;
+constexpr int kRayHit = 1;
+constexpr int kRayMiss = 2;
+constexpr int kRayOverflow = 3;
+
+// Two-level traversal as a resumable state machine: start() sets up a ray,
+// each step() does one visit (an internal node, an instance entry or a leaf
+// with the pops that follow it). The order of visits is fixed: depth first,
+// nearer child first (ties to child 0), popped entries skipped once their
+// entry distance exceeds the closest hit.
 template <bool OCCLUSION, bool COUNT>
-__device__ inline int traverse(const TlasView& T, const SceneView& S, V3 o, V3 d, float tmin, float tmax,
-                               uint32_t ray_flags, uint32_t cull_mask, uint32_t sbt_offset, HitRec& best, Stack& st,
-                               TravCounters& cnt) {
+struct Traverser {
     RayCtx world, obj;
-    ray_setup(world, o, d);
-    bool in_blas = false;
-    const Node* nodes = T.nodes;
-    const TriRec* tris = nullptr;
-    uint32_t inst_index = 0, inst_flags = 0;
-    float tcur = tmax;
-    int found = 0;
-    best.t = tmax;
-    best.u = best.v = 0.0f;
-    best.inst = best.prim = best.geom = RTG_NO_HIT;
-    st.sp = 0;
-    uint32_t cur = 0;
-    bool have = true;
-    while (true) {
-        if (have) {
-            const RayCtx& cr = in_blas ? obj : world;
-            if (!(cur & kLeafBit)) {
-                const float4* np = reinterpret_cast<const float4*>(nodes + cur);
-                float4 X = np[0], Y = np[1], Z = np[2];
-                uint4 C = reinterpret_cast<const uint4*>(np)[3];
-                if (COUNT) cnt.nodes++;
-                bool h0, h1;
-                float tn0, tn1;
-                box_test2(X, Y, Z, cr, tmin, tcur, h0, h1, tn0, tn1);
-                h0 = h0 && C.x != kEmpty;
-                h1 = h1 && C.y != kEmpty;
-                if (h0 && h1) {
-                    bool first1 = tn1 < tn0;
-                    cur = first1 ? C.y : C.x;
-                    if (st.sp >= kStackMax) return -1;
-                    st.push(first1 ? C.x : C.y, first1 ? tn0 : tn1);
-                } else if (h0) {
-                    cur = C.x;
-                } else if (h1) {
-                    cur = C.y;
-                } else {
-                    have = false;
-                }
-                continue;
-            }
-            if (!in_blas) {
-                const InstRec* rec = T.recs + (cur & kFirstMask);
-                have = false;
-                uint32_t mf = rec->mask_flags;
-                if (((mf & 0xffu) & cull_mask) == 0) continue;
-                const uint8_t* bb = reinterpret_cast<const uint8_t*>(rec->blas);
-                if (bb == nullptr) continue;
-                if (COUNT) cnt.insts++;
-                if (st.sp >= kStackMax) return -1;
-                st.push(kSentinel, -__builtin_inff());
-                float w2o[12];
-#pragma unroll
-                for (int k = 0; k < 12; ++k) w2o[k] = rec->w2o[k];
-                ray_setup(obj, m34_point(w2o, o), m34_dir(w2o, d));
-                const AsHeader* bh = reinterpret_cast<const AsHeader*>(bb);
-                nodes = reinterpret_cast<const Node*>(bb + bh->nodes_off);
-                tris = reinterpret_cast<const TriRec*>(bb + bh->prims_off);
-                inst_index = rec->index;
-                inst_flags = (mf >> 8) & 0xffu;
-                in_blas = true;
-                cur = 0;
-                have = true;
-                continue;
-            }
-            uint32_t first = cur & kFirstMask;
-            uint32_t count = ((cur >> 27) & 0xfu) + 1u;
-            for (uint32_t k = first; k < first + count; ++k) {
-                const float4* tp = reinterpret_cast<const float4*>(tris + k);
-                float4 a = tp[0], b = tp[1], c = tp[2];
-                if (COUNT) cnt.tris++;
-                float t, u, v;
-                if (!tri_test(obj, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), tmin, tcur, t, u, v))
-                    continue;
-                uint32_t prim = f2u(a.w), gword = f2u(b.w);
-                uint32_t geom = gword & ~kNonOpaqueBit;
-                if (!OCCLUSION &&
-                    !(t < best.t || (t == best.t && key_less(inst_index, geom, prim, best.inst, best.geom, best.prim))))
-                    continue;
-                if (!is_opaque(gword, inst_flags, ray_flags)) {
-                    uint32_t group = (T.src[inst_index].sbt_offset_and_flags & 0xffffffu) + sbt_offset;
-                    if (group < RTG_HIT_GROUP_COUNT) {
-                        if (COUNT) cnt.anyhits++;
-                        if (!alpha_clip_keep(S, T.src, inst_index, geom, prim, u, v)) continue;
-                    }
-                }
-                found = 1;
-                best.t = t;
-                best.u = u;
-                best.v = v;
-                best.inst = inst_index;
-                best.geom = geom;
-                best.prim = prim;
-                if (OCCLUSION || (ray_flags & RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT)) return 1;
-                tcur = t;
-            }
-            have = false;
-            continue;
-        }
-        // pop
+    const Node* nodes;
+    const TriRec* tris;
+    float tmin, tcur;
+    uint32_t ray_flags, cull_mask, sbt_offset;
+    uint32_t inst_index, inst_flags;
+    uint32_t cur;
+    bool in_blas;
+    int found;
+    HitRec best;
+
+    __device__ inline void start(const TlasView& T, V3 o, V3 d, float tmin_, float tmax_, uint32_t flags,
+                                 uint32_t mask, uint32_t sbt, Stack& st) {
+        ray_setup_box(world, o, d);
+        in_blas = false;
+        nodes = T.nodes;
+        tris = nullptr;
+        tmin = tmin_;
+        tcur = tmax_;
+        ray_flags = flags;
+        cull_mask = mask;
+        sbt_offset = sbt;
+        inst_index = inst_flags = 0;
+        found = 0;
+        best.t = tmax_;
+        best.u = best.v = 0.0f;
+        best.inst = best.prim = best.geom = RTG_NO_HIT;
+        st.sp = 0;
+        cur = 0;
+    }
+
+    __device__ inline int done() const { return found ? kRayHit : kRayMiss; }
+
+    // Next entry to visit; false once the stack is empty.
+    __device__ inline bool pop(const TlasView& T, Stack& st) {
         while (st.sp > 0) {
             uint2 e = st.pop();
             if (e.x == kSentinel) {
@@ -605,12 +592,159 @@ __device__ inline int traverse(const TlasView& T, const SceneView& S, V3 o, V3 d
             }
             if (u2f(e.y) > tcur) continue;
             cur = e.x;
-            have = true;
-            break;
+            return true;
         }
-        if (!have) break;
+        return false;
     }
-    return found;
+
+    __device__ inline int step(const TlasView& T, const SceneView& S, Stack& st, TravCounters& cnt) {
+        if (!(cur & kLeafBit)) {
+            const RayCtx& cr = in_blas ? obj : world;
+            const float4* np = reinterpret_cast<const float4*>(nodes + cur);
+            float4 X = gld_f4(np), Y = gld_f4(np + 1), Z = gld_f4(np + 2);
+            uint4 C = gld_u4(np + 3);
+            if (COUNT) cnt.nodes++;
+            bool h0, h1;
+            float tn0, tn1;
+            box_test2(X, Y, Z, cr, tmin, tcur, h0, h1, tn0, tn1);
+            h0 = h0 && C.x != kEmpty;
+            h1 = h1 && C.y != kEmpty;
+            if (h0 && h1) {
+                bool first1 = tn1 < tn0;
+                cur = first1 ? C.y : C.x;
+                if (st.sp >= kStackMax) return kRayOverflow;
+                st.push(first1 ? C.x : C.y, first1 ? tn0 : tn1);
+            } else if (h0) {
+                cur = C.x;
+            } else if (h1) {
+                cur = C.y;
+            } else if (!pop(T, st)) {
+                return done();
+            }
+            return kRayActive;
+        }
+        if (!in_blas) {
+            const InstRec* rec = T.recs + (cur & kFirstMask);
+            const float4* rp = reinterpret_cast<const float4*>(rec);
+            uint4 tail = gld_u4(rp + 3);  // w2o[8..11] is rp[2]; rp[3] = blas lo, blas hi, index, mask_flags
+            uint32_t mf = tail.w;
+            const uint8_t* bb = reinterpret_cast<const uint8_t*>((uint64_t)tail.x | ((uint64_t)tail.y << 32));
+            if (((mf & 0xffu) & cull_mask) == 0 || bb == nullptr) {
+                if (!pop(T, st)) return done();
+                return kRayActive;
+            }
+            if (COUNT) cnt.insts++;
+            if (st.sp >= kStackMax) return kRayOverflow;
+            st.push(kSentinel, -__builtin_inff());
+            float w2o[12];
+            float4 m0 = gld_f4(rp), m1 = gld_f4(rp + 1), m2 = gld_f4(rp + 2);
+            w2o[0] = m0.x; w2o[1] = m0.y; w2o[2] = m0.z; w2o[3] = m0.w;
+            w2o[4] = m1.x; w2o[5] = m1.y; w2o[6] = m1.z; w2o[7] = m1.w;
+            w2o[8] = m2.x; w2o[9] = m2.y; w2o[10] = m2.z; w2o[11] = m2.w;
+            ray_setup(obj, m34_point(w2o, world.o), m34_dir(w2o, world.d));
+            const AsHeader* bh = reinterpret_cast<const AsHeader*>(bb);
+            nodes = reinterpret_cast<const Node*>(bb + gld_u64(&bh->nodes_off));
+            tris = reinterpret_cast<const TriRec*>(bb + gld_u64(&bh->prims_off));
+            inst_index = tail.z;
+            inst_flags = (mf >> 8) & 0xffu;
+            in_blas = true;
+            cur = 0;
+            return kRayActive;
+        }
+        uint32_t first = cur & kFirstMask;
+        uint32_t count = ((cur >> 27) & 0xfu) + 1u;
+        for (uint32_t k = first; k < first + count; ++k) {
+            const float4* tp = reinterpret_cast<const float4*>(tris + k);
+            float4 a = gld_f4(tp), b = gld_f4(tp + 1), c = gld_f4(tp + 2);
+            if (COUNT) cnt.tris++;
+            float t, u, v;
+            if (!tri_test(obj, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), tmin, tcur, t, u, v))
+                continue;
+            uint32_t prim = f2u(a.w), gword = f2u(b.w);
+            uint32_t geom = gword & ~kNonOpaqueBit;
+            if (!OCCLUSION &&
+                !(t < best.t || (t == best.t && key_less(inst_index, geom, prim, best.inst, best.geom, best.prim))))
+                continue;
+            if (!is_opaque(gword, inst_flags, ray_flags)) {
+                uint32_t group = (gld_u32(&T.src[inst_index].sbt_offset_and_flags) & 0xffffffu) + sbt_offset;
+                if (group < RTG_HIT_GROUP_COUNT) {
+                    if (COUNT) cnt.anyhits++;
+                    if (!alpha_clip_keep(S, T.src, inst_index, geom, prim, u, v)) continue;
+                }
+            }
+            found = 1;
+            best.t = t;
+            best.u = u;
+            best.v = v;
+            best.inst = inst_index;
+            best.geom = geom;
+            best.prim = prim;
+            if (OCCLUSION || (ray_flags & RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT)) return kRayHit;
+            tcur = t;
+        }
+        if (!pop(T, st)) return done();
+        return kRayActive;
+    }
+};
+
+// Runs one ray to completion. Returns 1 for an accepted hit, 0 for none, -1
+// if the stack overflowed.
+template <bool OCCLUSION, bool COUNT>
+__device__ inline int traverse(const TlasView& T, const SceneView& S, V3 o, V3 d, float tmin, float tmax,
+                               uint32_t ray_flags, uint32_t cull_mask, uint32_t sbt_offset, HitRec& best, Stack& st,
+                               TravCounters& cnt) {
+    Traverser<OCCLUSION, COUNT> tr;
+    tr.start(T, o, d, tmin, tmax, ray_flags, cull_mask, sbt_offset, st);
+    int s;
+    while ((s = tr.step(T, S, st, cnt)) == kRayActive) {
+    }
+    best = tr.best;
+    return s == kRayHit ? 1 : (s == kRayOverflow ? -1 : 0);
 }
+
+// Hands queue entries to the idle lanes of a wavefront. The wave takes
+// chunks of `chunk` entries from a global counter (one atomic per chunk)
+// and serves its lanes from the chunk, so a lane that finished its ray gets
+// a new one without waiting for the rest of the wave.
+struct WaveFeeder {
+    uint32_t next, end;
+    uint32_t chunk;  // entries taken per atomic (multiple of 64)
+    bool exhausted;
+    __device__ inline void init(uint32_t chunk_) {
+        next = end = 0;
+        chunk = chunk_;
+        exhausted = false;
+    }
+    // `idle` is the wave's ballot of lanes asking for work (wave-uniform);
+    // returns this lane's entry or ~0u.
+    __device__ inline uint32_t feed(uint64_t idle, bool want, uint32_t* counter, uint32_t n) {
+        const int lane = threadIdx.x & 63;
+        const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        uint32_t rank = (uint32_t)__popcll(idle & lt);
+        uint32_t k = (uint32_t)__popcll(idle);
+        uint32_t mine = ~0u;
+        while (k > 0 && !exhausted) {
+            if (next >= end) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(counter, chunk);
+                base = (uint32_t)__shfl((int)base, 0);
+                if (base >= n) {
+                    exhausted = true;
+                    break;
+                }
+                next = base;
+                end = min(base + chunk, n);
+            }
+            uint32_t take = min(k, end - next);
+            if (want && mine == ~0u) {
+                if (rank < take) mine = next + rank;
+                else rank -= take;
+            }
+            next += take;
+            k -= take;
+        }
+        return mine;
+    }
+};
 
 }  // namespace rtg

@@ -80,6 +80,21 @@ struct TimedLaunch {
 
 struct FrameArena;  // trace.hip
 
+// Launch parameters of the traversal kernels (defaults, overridable through
+// the environment: RTG_REFILL_CLOSEST, RTG_REFILL_OCCLUSION (0 selects the
+// non-persistent kernels), RTG_PATH_ORDER, RTG_TRACE_BLOCKS_PER_CU,
+// RTG_FEED_CHUNK).
+// Measured on MI355X (1080p large scene): refilling lanes before the whole
+// wave is idle mixes incoherent rays and loses more than it gains, so the
+// default refills only an idle wave, one 64-ray batch per atomic.
+struct Tuning {
+    int refill_closest = 64;
+    int refill_occlusion = 64;
+    int path_order = 1;
+    int blocks_per_cu = 5;
+    int feed_chunk = 64;
+};
+
 }  // namespace rtg
 
 struct rtg_ctx {
@@ -100,6 +115,7 @@ struct rtg_ctx {
     double kernel_ms[RTG_KERNEL_COUNT] = {};
     uint64_t kernel_launches[RTG_KERNEL_COUNT] = {};
     rtg::FrameArena* frame = nullptr;
+    rtg::Tuning tune;
 };
 
 namespace rtg {

@@ -20,6 +20,42 @@
 namespace rtg {
 
 constexpr int kTraceThreads = 256;
+constexpr int kCountSlots = 16;
+constexpr int kFeedLevel = 8;    // counts[kFeedLevel + level]: feed counter of a trace level
+constexpr int kFeedShadow = 12;  // counts[kFeedShadow]: feed counter of the shadow queue
+
+// Order of an image's pixels along its paths: bands of 8 rows, each band
+// walked column by column, so that the 64 lanes of a wavefront cover an 8x8
+// block of pixels (coherent rays). A bijection of [0, W*H) for every W, H.
+// order 0 is plain row-major.
+__device__ inline void order_to_pixel(uint32_t t, uint32_t W, uint32_t H, uint32_t& x, uint32_t& y,
+                                      uint32_t order) {
+    if (order == 0) {
+        x = t % W;
+        y = t / W;
+        return;
+    }
+    uint32_t full = (H / 8) * 8;
+    uint32_t fb = full * W;
+    if (t < fb) {
+        uint32_t band = t / (8 * W);
+        uint32_t i = t - band * 8 * W;
+        x = i / 8;
+        y = band * 8 + (i & 7);
+    } else {
+        uint32_t h = H - full;
+        uint32_t i = t - fb;
+        x = i / h;
+        y = full + i % h;
+    }
+}
+__device__ inline uint32_t pixel_to_order(uint32_t x, uint32_t y, uint32_t W, uint32_t H, uint32_t order) {
+    if (order == 0) return y * W + x;
+    uint32_t full = (H / 8) * 8;
+    if (y < full) return (y / 8) * 8 * W + x * 8 + (y & 7);
+    uint32_t h = H - full;
+    return full * W + x * h + (y - full);
+}
 
 struct JobDev {
     const uint8_t* tlas;
@@ -57,9 +93,14 @@ struct Frame {
     uint4* hit_a;        // t bits, instance, primitive, geometry
     float2* hit_b;       // u, v
     Queue sq;            // shadow rays
-    uint32_t* counts;    // [0..3] trace queue sizes per trace level, [4] shadow rays
+    uint32_t* counts;    // [0..3] queue sizes per trace level, [4] shadow rays,
+                         // [8..11] feed counters per trace level, [12] shadow feed counter
     uint64_t* counters;  // rtg_trace_counters
     uint2* spill;
+    uint32_t order;           // pixel order along paths (0 row-major, 1 8x8 bands)
+    int refill_closest;       // idle lanes that trigger a refill (0: rounds kernel)
+    int refill_occlusion;
+    uint32_t feed_chunk;      // queue entries a wave takes per atomic
 };
 
 __device__ inline uint32_t find_job(const JobDev* jobs, uint32_t njobs, uint64_t p, bool by_pixel) {
@@ -108,6 +149,13 @@ __device__ inline void flush_counters(uint64_t* counters, const TravCounters& c,
     wave_add_counter(&counters[base + 3], c.anyhits);
 }
 
+// Lanes of a persistent traversal kernel keep tracing rays until the queue is
+// empty: a lane that finishes asks the wave's feeder for the next entry once
+// at least `refill_min` lanes are idle (or all are).
+__device__ inline bool want_refill(uint64_t idle, const WaveFeeder& f, int refill_min) {
+    return idle != 0 && !f.exhausted && (__popcll(idle) >= refill_min || idle == ~0ull);
+}
+
 // ---------------------------------------------------------------------------
 // Ray queries: rtg_trace_rays
 // ---------------------------------------------------------------------------
@@ -115,7 +163,9 @@ template <bool OCC, bool COUNT>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_rays(const uint8_t* tlas, SceneView S, const rtg_ray* rays,
                                                               uint32_t n, uint32_t ray_flags, uint32_t cull_mask,
                                                               uint32_t sbt_offset, rtg_hit* hits, uint32_t* occ,
-                                                              uint2* spill, uint64_t* counters, uint32_t* status) {
+                                                              uint2* spill, uint32_t* feed_counter,
+                                                              uint64_t* counters, uint32_t* status, int refill_min,
+                                                              uint32_t feed_chunk) {
     __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
     const TlasView T = tlas_view(tlas);
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
@@ -124,28 +174,42 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_rays(const uint8_t* tla
     st.stride = kTraceThreads;
     st.spill = spill + (size_t)gtid * kSpillDepth;
     TravCounters cnt = {0, 0, 0, 0};
-    const uint32_t stride = gridDim.x * kTraceThreads;
-    const uint32_t rounds = (n + stride - 1) / stride;
-    for (uint32_t r = 0; r < rounds; ++r) {
-        uint32_t i = r * stride + gtid;
-        if (i < n) {
-            const float4* rp = reinterpret_cast<const float4*>(rays + i);
-            float4 a = rp[0], b = rp[1];
-            HitRec h;
-            int res = traverse<OCC, COUNT>(T, S, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, ray_flags, cull_mask,
-                                           sbt_offset, h, st, cnt);
-            if (res < 0) atomicOr(status, (uint32_t)kStatusStackOverflow);
-            if (OCC) {
-                occ[i] = res > 0 ? 1u : 0u;
-            } else {
-                rtg_hit o;
-                o.t = h.t;
-                o.instance = h.inst;
-                o.primitive = h.prim;
-                o.geometry = h.geom;
-                o.u = h.u;
-                o.v = h.v;
-                hits[i] = o;
+    Traverser<OCC, COUNT> tr;
+    WaveFeeder feeder;
+    feeder.init(feed_chunk);
+    bool active = false;
+    uint32_t ray = 0;
+    while (true) {
+        uint64_t idle = __ballot(!active);
+        if (want_refill(idle, feeder, refill_min)) {
+            uint32_t id = feeder.feed(idle, !active, feed_counter, n);
+            if (id != ~0u) {
+                const float4* rp = reinterpret_cast<const float4*>(rays + id);
+                float4 a = rp[0], b = rp[1];
+                tr.start(T, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, ray_flags, cull_mask, sbt_offset, st);
+                ray = id;
+                active = true;
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (active) {
+            int res = tr.step(T, S, st, cnt);
+            if (res != kRayActive) {
+                if (res == kRayOverflow) atomicOr(status, (uint32_t)kStatusStackOverflow);
+                if (OCC) {
+                    occ[ray] = res == kRayHit ? 1u : 0u;
+                } else {
+                    const HitRec& h = tr.best;
+                    rtg_hit o;
+                    o.t = h.t;
+                    o.instance = h.inst;
+                    o.primitive = h.prim;
+                    o.geometry = h.geom;
+                    o.u = h.u;
+                    o.v = h.v;
+                    hits[ray] = o;
+                }
+                active = false;
             }
         }
     }
@@ -156,8 +220,12 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_rays(const uint8_t* tla
 // Wavefront pipeline
 // ---------------------------------------------------------------------------
 __global__ void k_frame_begin(uint32_t* counts, uint64_t* counters, bool reset_counters) {
-    if (threadIdx.x < 8) counts[threadIdx.x] = 0;
+    if (threadIdx.x < kCountSlots) counts[threadIdx.x] = 0;
     if (reset_counters && threadIdx.x < kCounterSlots) counters[threadIdx.x] = 0;
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
 __global__ void k_reset_counters(uint64_t* counters, uint64_t primary) {
@@ -174,8 +242,8 @@ __global__ __launch_bounds__(256) void k_raygen(Frame F, bool count) {
         const JobDev& J = F.jobs[j];
         uint64_t local = p - J.sample_base;
         uint32_t s = (uint32_t)(local % J.spp);
-        uint32_t pix = (uint32_t)(local / J.spp);
-        uint32_t px = pix % J.width, py = pix / J.width;
+        uint32_t px, py;
+        order_to_pixel((uint32_t)(local / J.spp), J.width, J.height, px, py, F.order);
         float jx = 0.5f, jy = 0.5f;
         if (J.spp > 1) {
             jx = fractf(0.5f + (float)s * 0.754877669f);
@@ -216,22 +284,71 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int le
     st.stride = kTraceThreads;
     st.spill = F.spill + (size_t)gtid * kSpillDepth;
     TravCounters cnt = {0, 0, 0, 0};
+    Traverser<false, COUNT> tr;
+    WaveFeeder feeder;
+    feeder.init(F.feed_chunk);
+    bool active = false;
+    uint32_t slot = 0, path = 0;
+    const JobDev* J = nullptr;
+    TlasView T;
+    uint64_t c0 = 0;
+    while (true) {
+        uint64_t idle = __ballot(!active);
+        if (want_refill(idle, feeder, F.refill_closest)) {
+            uint32_t id = feeder.feed(idle, !active, &F.counts[kFeedLevel + level], n);
+            if (id != ~0u) {
+                slot = id;
+                path = Q.path[id];
+                J = &F.jobs[find_job(F.jobs, F.njobs, path, false)];
+                T = tlas_view(J->tlas);
+                float4 a = Q.o[id], b = Q.d[id];
+                tr.start(T, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, RTG_RAY_FLAG_NONE, 0xffu, 0u, st);
+                if (J->flags & RTG_TRACE_HEATMAP) c0 = __builtin_readcyclecounter();
+                active = true;
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (active) {
+            int res = tr.step(T, J->scene, st, cnt);
+            if (res != kRayActive) {
+                if (res == kRayOverflow) atomicOr(status, (uint32_t)kStatusStackOverflow);
+                if (J->flags & RTG_TRACE_HEATMAP) atomicAdd(&F.cycles[path], (uint32_t)(__builtin_readcyclecounter() - c0));
+                const HitRec& h = tr.best;
+                F.hit_a[slot] = make_uint4(f2u(h.t), h.inst, h.prim, h.geom);
+                F.hit_b[slot] = make_float2(h.u, h.v);
+                active = false;
+            }
+        }
+    }
+    if (COUNT) flush_counters(F.counters, cnt, kCountersClosest);
+}
+
+// Non-persistent variant: lanes take rays in grid-stride rounds and run each
+// to completion (a wave waits for its slowest lane).
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceThreads) void k_trace_closest_rounds(Frame F, int level, const uint32_t* count_ptr,
+                                                                        uint32_t fixed_count, uint32_t* status) {
+    __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
+    const uint32_t n = count_ptr ? *count_ptr : fixed_count;
+    const Queue Q = F.q[level & 1];
+    const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
+    Stack st;
+    st.lds = s_stack + threadIdx.x;
+    st.stride = kTraceThreads;
+    st.spill = F.spill + (size_t)gtid * kSpillDepth;
+    TravCounters cnt = {0, 0, 0, 0};
     const uint32_t stride = gridDim.x * kTraceThreads;
     const uint32_t rounds = (n + stride - 1) / stride;
     for (uint32_t r = 0; r < rounds; ++r) {
         uint32_t i = r * stride + gtid;
         if (i < n) {
             uint32_t p = Q.path[i];
-            uint32_t j = find_job(F.jobs, F.njobs, p, false);
-            const JobDev& J = F.jobs[j];
+            const JobDev& J = F.jobs[find_job(F.jobs, F.njobs, p, false)];
             const TlasView T = tlas_view(J.tlas);
             float4 a = Q.o[i], b = Q.d[i];
-            bool heat = (J.flags & RTG_TRACE_HEATMAP) != 0;
-            uint64_t c0 = heat ? __builtin_readcyclecounter() : 0;
             HitRec h;
             int res = traverse<false, COUNT>(T, J.scene, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w,
                                              RTG_RAY_FLAG_NONE, 0xffu, 0u, h, st, cnt);
-            if (heat) atomicAdd(&F.cycles[p], (uint32_t)(__builtin_readcyclecounter() - c0));
             if (res < 0) atomicOr(status, (uint32_t)kStatusStackOverflow);
             F.hit_a[i] = make_uint4(f2u(h.t), h.inst, h.prim, h.geom);
             F.hit_b[i] = make_float2(h.u, h.v);
@@ -242,11 +359,35 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int le
 
 __device__ inline V3 ld3(uint64_t base, uint32_t i) {
     const float* p = reinterpret_cast<const float*>(base) + 3ull * i;
-    return v3(p[0], p[1], p[2]);
+    return v3(gld_f32(p), gld_f32(p + 1), gld_f32(p + 2));
 }
 __device__ inline V2 ld2(uint64_t base, uint32_t i) {
     const float* p = reinterpret_cast<const float*>(base) + 2ull * i;
-    return V2{p[0], p[1]};
+    return V2{gld_f32(p), gld_f32(p + 1)};
+}
+// ModelInfo / GeometryInfo / index-buffer reads of the hit shaders
+// (hit_shader_common.glsl:83-154)
+struct HitGeom {
+    uint64_t pos, nrm, uv;
+    uint32_t i0, i1, i2;
+    uint32_t diffuse, mr;
+    int32_t normal_map;
+};
+__device__ inline HitGeom load_hit_geom(const SceneView& S, uint32_t custom, uint32_t geom, uint32_t prim) {
+    const rtg_model_info* mi = S.models + custom;
+    HitGeom g;
+    g.pos = gld_u64(&mi->position_buffer_address);
+    g.nrm = gld_u64(&mi->normal_buffer_address);
+    g.uv = gld_u64(&mi->uv_buffer_address);
+    const rtg_geometry_info* gi = reinterpret_cast<const rtg_geometry_info*>(gld_u64(&mi->geometry_info_address)) + geom;
+    const uint32_t* idx = reinterpret_cast<const uint32_t*>(gld_u64(&gi->index_buffer_address)) + 3ull * prim;
+    g.i0 = gld_u32(idx);
+    g.i1 = gld_u32(idx + 1);
+    g.i2 = gld_u32(idx + 2);
+    g.diffuse = gld_u32(&gi->diffuse_image_index);
+    g.mr = gld_u32(&gi->metallic_roughness_image_index);
+    g.normal_map = (int32_t)gld_u32(&gi->normal_map_image_index);
+    return g;
 }
 
 // sample_directional_light (closest_hit_textured.glsl:77-94)
@@ -318,10 +459,9 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
             p = Q.path[i];
             J = &F.jobs[find_job(F.jobs, F.njobs, p, false)];
             uint64_t local = p - J->sample_base;
-            uint32_t pix = (uint32_t)(local / J->spp);
             uint32_t s = (uint32_t)(local % J->spp);
-            px = pix % J->width;
-            py = pix / J->width;
+            order_to_pixel((uint32_t)(local / J->spp), J->width, J->height, px, py, F.order);
+            uint64_t out_index = ((uint64_t)py * J->width + px) * J->spp + s;
             frame = J->frame_index * J->spp + s;
             sun = v3(J->sun[0], J->sun[1], J->sun[2]);
             sun_radius = J->sun_radius;
@@ -340,7 +480,7 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
                 oh.geometry = geom;
                 oh.u = hb.x;
                 oh.v = hb.y;
-                J->out_hits[local] = oh;
+                J->out_hits[out_index] = oh;
             }
             const SceneView& S = J->scene;
             if (inst == RTG_NO_HIT) {
@@ -350,27 +490,22 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
                 F.path_rgb[p] = make_float4(c.x, c.y, c.z, 0.0f);
             } else {
                 const TlasView T = tlas_view(J->tlas);
-                const rtg_instance& src = T.src[inst];
-                uint32_t group = src.sbt_offset_and_flags & 0xffffffu;
-                uint32_t custom = src.custom_index_and_mask & 0xffffffu;
-                float o2w[12];
-#pragma unroll
-                for (int k = 0; k < 12; ++k) o2w[k] = (&src.transform[0][0])[k];
+                const float4* sp = reinterpret_cast<const float4*>(T.src + inst);
+                float4 r0 = gld_f4(sp), r1 = gld_f4(sp + 1), r2 = gld_f4(sp + 2);
+                uint4 tail = gld_u4(sp + 3);  // custom|mask, sbt|flags, blas
+                uint32_t group = tail.y & 0xffffffu;
+                uint32_t custom = tail.x & 0xffffffu;
+                float o2w[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
                 V3 bc = v3((1.0f - hb.x) - hb.y, hb.x, hb.y);
                 if ((group == RTG_HIT_TEXTURED || group == RTG_HIT_MIRROR) && custom >= S.model_count) {
                     // no model: colour stays 0, no continuation
                 } else if (group == RTG_HIT_TEXTURED) {
                     // closest_hit_textured (closest_hit_textured.glsl:174-226)
-                    const rtg_model_info mi = S.models[custom];
-                    const rtg_geometry_info gi = reinterpret_cast<const rtg_geometry_info*>(mi.geometry_info_address)[geom];
-                    const uint32_t* idx = reinterpret_cast<const uint32_t*>(gi.index_buffer_address) + 3ull * prim;
-                    uint32_t i0 = idx[0], i1 = idx[1], i2 = idx[2];
-                    V3 pa = ld3(mi.position_buffer_address, i0), pb = ld3(mi.position_buffer_address, i1),
-                       pc = ld3(mi.position_buffer_address, i2);
-                    V3 na = ld3(mi.normal_buffer_address, i0), nb = ld3(mi.normal_buffer_address, i1),
-                       nc = ld3(mi.normal_buffer_address, i2);
-                    V2 ua = ld2(mi.uv_buffer_address, i0), ub = ld2(mi.uv_buffer_address, i1),
-                       uc = ld2(mi.uv_buffer_address, i2);
+                    const HitGeom gi = load_hit_geom(S, custom, geom, prim);
+                    uint32_t i0 = gi.i0, i1 = gi.i1, i2 = gi.i2;
+                    V3 pa = ld3(gi.pos, i0), pb = ld3(gi.pos, i1), pc = ld3(gi.pos, i2);
+                    V3 na = ld3(gi.nrm, i0), nb = ld3(gi.nrm, i1), nc = ld3(gi.nrm, i2);
+                    V2 ua = ld2(gi.uv, i0), ub = ld2(gi.uv, i1), uc = ld2(gi.uv, i2);
                     V3 pos = interp3(pa, pb, pc, bc);
                     V3 nrm = interp3(na, nb, nc, bc);
                     V2 uv = interp2(ua, ub, uc, bc);
@@ -387,18 +522,18 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
                     sh_origin = m34_point(o2w, fixed);
                     want_shadow = J->shadow_rays;
                     // read_material_from_textures (:47-63)
-                    float4 dt = sample_index(S, gi.diffuse_image_index, uv.x, uv.y);
-                    float4 mr = sample_index(S, gi.metallic_roughness_image_index, uv.x, uv.y);
+                    float4 dt = sample_index(S, gi.diffuse, uv.x, uv.y);
+                    float4 mr = sample_index(S, gi.mr, uv.x, uv.y);
                     V3 base = v3(dt.x, dt.y, dt.z);
                     float metallic = mr.z, roughness = mr.y;
                     // calculate_normal (:141-157)
                     float w2o[12];
                     m34_inverse(o2w, w2o);
                     V3 normal;
-                    if (gi.normal_map_image_index < 0) {
+                    if (gi.normal_map < 0) {
                         normal = normalize(m34_tdir(w2o, nrm));
                     } else {
-                        float4 mt = sample_index(S, (uint32_t)gi.normal_map_image_index, uv.x, uv.y);
+                        float4 mt = sample_index(S, (uint32_t)gi.normal_map, uv.x, uv.y);
                         V3 mapn = v3(mt.x * 2.0f - 1.0f, mt.y * 2.0f - 1.0f, mt.z * 2.0f - 1.0f);
                         V3 dp1 = pb - pa, dp2 = pc - pa;
                         float du1x = ub.x - ua.x, du1y = ub.y - ua.y, du2x = uc.x - ua.x, du2y = uc.y - ua.y;
@@ -418,11 +553,8 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
                     F.shade_b[p] = make_float4(0.1f * base.x, 0.1f * base.y, 0.1f * base.z, 1.0f);
                 } else if (group == RTG_HIT_MIRROR) {
                     // closest_hit_mirror (closest_hit_mirror.glsl:11-29)
-                    const rtg_model_info mi = S.models[custom];
-                    const rtg_geometry_info gi = reinterpret_cast<const rtg_geometry_info*>(mi.geometry_info_address)[geom];
-                    const uint32_t* idx = reinterpret_cast<const uint32_t*>(gi.index_buffer_address) + 3ull * prim;
-                    V3 na = ld3(mi.normal_buffer_address, idx[0]), nb = ld3(mi.normal_buffer_address, idx[1]),
-                       nc = ld3(mi.normal_buffer_address, idx[2]);
+                    const HitGeom gi = load_hit_geom(S, custom, geom, prim);
+                    V3 na = ld3(gi.nrm, gi.i0), nb = ld3(gi.nrm, gi.i1), nc = ld3(gi.nrm, gi.i2);
                     V3 ni = interp3(na, nb, nc, bc);
                     float w2o[12];
                     m34_inverse(o2w, w2o);
@@ -486,6 +618,54 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion(Frame F, uint
     st.stride = kTraceThreads;
     st.spill = F.spill + (size_t)gtid * kSpillDepth;
     TravCounters cnt = {0, 0, 0, 0};
+    Traverser<true, COUNT> tr;
+    WaveFeeder feeder;
+    feeder.init(F.feed_chunk);
+    bool active = false;
+    uint32_t path = 0;
+    const JobDev* J = nullptr;
+    TlasView T;
+    uint64_t c0 = 0;
+    while (true) {
+        uint64_t idle = __ballot(!active);
+        if (want_refill(idle, feeder, F.refill_occlusion)) {
+            uint32_t id = feeder.feed(idle, !active, &F.counts[kFeedShadow], n);
+            if (id != ~0u) {
+                path = F.sq.path[id];
+                J = &F.jobs[find_job(F.jobs, F.njobs, path, false)];
+                T = tlas_view(J->tlas);
+                float4 a = F.sq.o[id], b = F.sq.d[id];
+                tr.start(T, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w,
+                         RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RTG_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER, 0xffu, 1u, st);
+                if (J->flags & RTG_TRACE_HEATMAP) c0 = __builtin_readcyclecounter();
+                active = true;
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (active) {
+            int res = tr.step(T, J->scene, st, cnt);
+            if (res != kRayActive) {
+                if (res == kRayOverflow) atomicOr(status, (uint32_t)kStatusStackOverflow);
+                if (J->flags & RTG_TRACE_HEATMAP) atomicAdd(&F.cycles[path], (uint32_t)(__builtin_readcyclecounter() - c0));
+                // shadow_ray_miss (lib.rs:33-36): a miss counts as visible
+                if (res == kRayMiss) atomicAdd(&F.vis[path], 1u);
+                active = false;
+            }
+        }
+    }
+    if (COUNT) flush_counters(F.counters, cnt, kCountersOcclusion);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion_rounds(Frame F, uint32_t* status) {
+    __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
+    const uint32_t n = F.counts[4];
+    const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
+    Stack st;
+    st.lds = s_stack + threadIdx.x;
+    st.stride = kTraceThreads;
+    st.spill = F.spill + (size_t)gtid * kSpillDepth;
+    TravCounters cnt = {0, 0, 0, 0};
     const uint32_t stride = gridDim.x * kTraceThreads;
     const uint32_t rounds = (n + stride - 1) / stride;
     for (uint32_t r = 0; r < rounds; ++r) {
@@ -495,13 +675,10 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion(Frame F, uint
             const JobDev& J = F.jobs[find_job(F.jobs, F.njobs, p, false)];
             const TlasView T = tlas_view(J.tlas);
             float4 a = F.sq.o[i], b = F.sq.d[i];
-            bool heat = (J.flags & RTG_TRACE_HEATMAP) != 0;
-            uint64_t c0 = heat ? __builtin_readcyclecounter() : 0;
             HitRec h;
             int res = traverse<true, COUNT>(T, J.scene, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w,
                                             RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RTG_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER,
                                             0xffu, 1u, h, st, cnt);
-            if (heat) atomicAdd(&F.cycles[p], (uint32_t)(__builtin_readcyclecounter() - c0));
             if (res < 0) atomicOr(status, (uint32_t)kStatusStackOverflow);
             if (res == 0) atomicAdd(&F.vis[p], 1u);
         }
@@ -547,10 +724,11 @@ __global__ __launch_bounds__(256) void k_resolve(Frame F, uint64_t total_pixels)
     for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < total_pixels; q += stride) {
         const JobDev& J = F.jobs[find_job(F.jobs, F.njobs, q, true)];
         uint32_t pix = (uint32_t)(q - J.pixel_base);
+        uint32_t t = pixel_to_order(pix % J.width, pix / J.width, J.width, J.height, F.order);
         V3 acc = v3(0.0f, 0.0f, 0.0f);
         uint32_t cycles = 0;
         for (uint32_t s = 0; s < J.spp; ++s) {
-            uint64_t p = J.sample_base + (uint64_t)pix * J.spp + s;
+            uint64_t p = J.sample_base + (uint64_t)t * J.spp + s;
             float4 b = F.shade_b[p];
             V3 c;
             if (b.w != 0.0f) {
@@ -591,6 +769,7 @@ __global__ __launch_bounds__(256) void k_resolve(Frame F, uint64_t total_pixels)
 struct FrameArena {
     DeviceBuffer buf;
     DeviceBuffer spill;
+    DeviceBuffer feed;  // feed counter of rtg_trace_rays
     uint64_t paths_cap = 0, shadow_cap = 0;
     uint32_t jobs_cap = 0;
     JobDev* h_jobs = nullptr;  // pinned staging
@@ -606,6 +785,7 @@ void free_frame_arena(rtg_ctx* ctx) {
     FrameArena* a = ctx->frame;
     a->buf.release();
     a->spill.release();
+    a->feed.release();
     if (a->h_jobs) hipHostFree(a->h_jobs);
     if (a->staging_free) hipEventDestroy(a->staging_free);
     delete a;
@@ -613,13 +793,15 @@ void free_frame_arena(rtg_ctx* ctx) {
 }
 
 static int trace_grid(rtg_ctx* ctx) {
-    // LDS stack of 32 KB per block: 5 blocks per CU are resident.
-    return ctx->cu_count * 5;
+    // persistent grid: blocks_per_cu blocks of 256 lanes per CU (the LDS
+    // stack takes 32 KB per block, so at most 5 are resident)
+    return ctx->cu_count * ctx->tune.blocks_per_cu;
 }
 
 static int ensure_spill(rtg_ctx* ctx, FrameArena* a) {
     size_t need = (size_t)trace_grid(ctx) * kTraceThreads * kSpillDepth * sizeof(uint2);
-    return a->spill.ensure(need);
+    if (a->spill.ensure(need) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
+    return a->feed.ensure(256);
 }
 
 static FrameArena* arena(rtg_ctx* ctx) {
@@ -649,7 +831,7 @@ static int ensure_frame(rtg_ctx* ctx, uint64_t paths, uint64_t shadow, uint32_t 
         carve_size<float4>(off, S);
         carve_size<float4>(off, S);
         carve_size<uint32_t>(off, S);
-        carve_size<uint32_t>(off, 8);
+        carve_size<uint32_t>(off, kCountSlots);
         carve_size<JobDev>(off, NJ);
         if (a->buf.ensure(off + 4096) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
         Carve c(a->buf.ptr);
@@ -669,7 +851,7 @@ static int ensure_frame(rtg_ctx* ctx, uint64_t paths, uint64_t shadow, uint32_t 
         F.sq.o = c.take<float4>(S);
         F.sq.d = c.take<float4>(S);
         F.sq.path = c.take<uint32_t>(S);
-        a->d_counts = c.take<uint32_t>(8);
+        a->d_counts = c.take<uint32_t>(kCountSlots);
         a->d_jobs = c.take<JobDev>(NJ);
         a->paths_cap = P;
         a->shadow_cap = S;
@@ -768,6 +950,10 @@ int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStrea
     F.counts = a->d_counts;
     F.counters = ctx->d_counters;
     F.spill = reinterpret_cast<uint2*>(a->spill.ptr);
+    F.order = (uint32_t)ctx->tune.path_order;
+    F.refill_closest = ctx->tune.refill_closest;
+    F.refill_occlusion = ctx->tune.refill_occlusion;
+    F.feed_chunk = (uint32_t)ctx->tune.feed_chunk;
     const bool cnt = ctx->counters_enabled;
     const unsigned G = (unsigned)trace_grid(ctx);
     const unsigned Gsmall = (unsigned)std::min<uint64_t>((paths + 255) / 256, (uint64_t)ctx->cu_count * 16);
@@ -781,12 +967,20 @@ int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStrea
         uint32_t fixed = level == 0 ? (uint32_t)paths : 0u;
         {
             LaunchTimer t(ctx, RTG_KERNEL_TRACE_CLOSEST, s);
-            if (cnt)
+            if (ctx->tune.refill_closest == 0) {
+                if (cnt)
+                    hipLaunchKernelGGL(k_trace_closest_rounds<true>, dim3(G), dim3(kTraceThreads), 0, s, F,
+                                       (int)level, cp, fixed, ctx->d_status);
+                else
+                    hipLaunchKernelGGL(k_trace_closest_rounds<false>, dim3(G), dim3(kTraceThreads), 0, s, F,
+                                       (int)level, cp, fixed, ctx->d_status);
+            } else if (cnt) {
                 hipLaunchKernelGGL(k_trace_closest<true>, dim3(G), dim3(kTraceThreads), 0, s, F, (int)level, cp, fixed,
                                    ctx->d_status);
-            else
+            } else {
                 hipLaunchKernelGGL(k_trace_closest<false>, dim3(G), dim3(kTraceThreads), 0, s, F, (int)level, cp,
                                    fixed, ctx->d_status);
+            }
         }
         {
             LaunchTimer t(ctx, RTG_KERNEL_SHADE, s);
@@ -798,10 +992,17 @@ int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStrea
     }
     {
         LaunchTimer t(ctx, RTG_KERNEL_TRACE_OCCLUSION, s);
-        if (cnt)
+        if (ctx->tune.refill_occlusion == 0) {
+            if (cnt)
+                hipLaunchKernelGGL(k_trace_occlusion_rounds<true>, dim3(G), dim3(kTraceThreads), 0, s, F, ctx->d_status);
+            else
+                hipLaunchKernelGGL(k_trace_occlusion_rounds<false>, dim3(G), dim3(kTraceThreads), 0, s, F,
+                                   ctx->d_status);
+        } else if (cnt) {
             hipLaunchKernelGGL(k_trace_occlusion<true>, dim3(G), dim3(kTraceThreads), 0, s, F, ctx->d_status);
-        else
+        } else {
             hipLaunchKernelGGL(k_trace_occlusion<false>, dim3(G), dim3(kTraceThreads), 0, s, F, ctx->d_status);
+        }
     }
     {
         LaunchTimer t(ctx, RTG_KERNEL_RESOLVE, s);
@@ -862,13 +1063,16 @@ extern "C" int rtg_trace_rays(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindi
     const unsigned G = (unsigned)std::min<uint64_t>((count + kTraceThreads - 1) / kTraceThreads, trace_grid(ctx));
     uint8_t* tl = reinterpret_cast<uint8_t*>(tlas);
     uint2* spill = reinterpret_cast<uint2*>(a->spill.ptr);
+    uint32_t* feed = reinterpret_cast<uint32_t*>(a->feed.ptr);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, feed, 1u);
     // the primary ray count of a ray query is the query size
     if (cnt) hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, ctx->d_counters, (uint64_t)count);
     LaunchTimer t(ctx, mode == RTG_QUERY_CLOSEST ? RTG_KERNEL_TRACE_CLOSEST : RTG_KERNEL_TRACE_OCCLUSION, s);
     cull_mask &= 0xffu;
 #define RTG_LAUNCH_RAYS(OCC, CNT)                                                                                   \
     hipLaunchKernelGGL((k_trace_rays<OCC, CNT>), dim3(G), dim3(kTraceThreads), 0, s, tl, S, rays, count, ray_flags, \
-                       cull_mask, sbt_offset, hits, occluded, spill, ctx->d_counters, ctx->d_status)
+                       cull_mask, sbt_offset, hits, occluded, spill, feed, ctx->d_counters, ctx->d_status,      \
+                       OCC ? ctx->tune.refill_occlusion : ctx->tune.refill_closest, (uint32_t)ctx->tune.feed_chunk)
     if (mode == RTG_QUERY_CLOSEST) {
         if (cnt) RTG_LAUNCH_RAYS(false, true);
         else RTG_LAUNCH_RAYS(false, false);
