@@ -49,6 +49,98 @@ def cpu_baseline(scene, threads):
     return hs, (rgba, rgb, hits), rays / dt / 1e6, dt
 
 
+def serving_benchmark(ctx, n_scenes=32, n_requests=160, seed=0x5EED0004):
+    """Served stream: n_scenes small-tori scenes resident in HBM; requests of
+    256x256, 1280x720 or 1920x1080 with 1-8 samples per pixel, each with a
+    random instance-transform update (a refit of a private TLAS copy).
+    Plain = one request per launch, one at a time; served = continuous
+    batching (up to 32 requests per batch, 2 batches in flight). Returns
+    saturation throughput (all requests at t=0) and p50/p99 latency at a
+    Poisson arrival rate of 0.6x the plain path's saturation throughput."""
+    import numpy as np
+    import torch
+    from rtg import scenes
+    from rtg.device import DeviceScene
+    from rtg.serve import Server, make_request
+
+    t0 = time.perf_counter()
+    resident = [DeviceScene(ctx, scenes.small_scene(seed=seed + 1 + k, width=256, height=256))
+                for k in range(n_scenes)]
+    rng = scenes.Rng(seed)
+    sizes = [(256, 256), (1280, 720), (1920, 1080)]
+    reqs, keep = [], []
+    paths = 0
+    for i in range(n_requests):
+        sid = int(rng.integers(1, 0, n_scenes)[0])
+        ds = resident[sid]
+        w, h = sizes[int(rng.integers(1, 0, 3)[0])]
+        spp = int(rng.integers(1, 1, 9)[0])
+        xf = [ins.transform.copy() for ins in ds.scene.instances]
+        rot = rng.uniform(3 * len(xf)).reshape(-1, 3)
+        for k in range(1, len(xf)):
+            xf[k] = scenes.affine(xf[k][:, 3], scenes.rotation_from_uniform(*rot[k]), 1.0)
+        buf = ds.instance_buffer(xf)
+        out = torch.empty(w * h, dtype=torch.int32, device="cuda:0")
+        r = make_request(sid, ds, w, h, spp, 1, 3, frame=i, instances=buf, outputs=(out, None, None), tag=i)
+        reqs.append(r)
+        keep.append((buf, out))
+        paths += w * h * spp
+    torch.cuda.synchronize()
+    log(f"serving setup {time.perf_counter() - t0:.1f}s: {n_scenes} scenes, {n_requests} requests, "
+        f"{paths / 1e6:.1f} M samples")
+
+    def run(max_batch, max_inflight, arrivals):
+        srv = Server(ctx, budget_bytes=48 << 30, max_batch=max_batch, max_inflight=max_inflight)
+        for ds in resident:
+            srv.add_scene(ds)
+        comp = srv.replay(reqs[:len(arrivals)], arrivals)
+        stats = srv.stats()
+        srv.close()
+        assert all(c.status == 0 for c in comp)
+        lat = np.array([c.done_s - c.submit_s for c in comp]) * 1e3
+        span = max(c.done_s for c in comp) - min(c.submit_s for c in comp)
+        return lat, span, stats
+
+    modes = {"plain": (1, 1), "served": (32, 2)}
+    # warm both paths (arenas, code objects)
+    for mb, mi in modes.values():
+        run(mb, mi, np.zeros(min(16, n_requests)))
+    sat = {}
+    for name, (mb, mi) in modes.items():
+        lat, span, stats = run(mb, mi, np.zeros(n_requests))
+        sat[name] = {"throughput_rps": round(n_requests / span, 2), "makespan_s": round(span, 3),
+                     "max_batch": stats["max_batch"], "batches": stats["batches"]}
+    rate = 0.6 * sat["plain"]["throughput_rps"]
+    arrivals = np.cumsum(-np.log(1.0 - scenes.Rng(seed + 99).uniform(n_requests)) / rate)
+    poisson = {}
+    for name, (mb, mi) in modes.items():
+        lat, span, stats = run(mb, mi, arrivals)
+        poisson[name] = {"p50_ms": round(float(np.percentile(lat, 50)), 2),
+                         "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                         "throughput_rps": round(n_requests / span, 2), "max_batch": stats["max_batch"]}
+    # exact ray count of the request set (counters of a plain replay)
+    ctx.set_counters(True)
+    rays = 0
+    srv = Server(ctx, budget_bytes=48 << 30, max_batch=1, max_inflight=1)
+    for ds in resident:
+        srv.add_scene(ds)
+    for r in reqs:
+        st, _ = srv.submit(r)
+        srv.poll(wait_all=True)
+        c = ctx.counters()
+        rays += c.primary_rays + c.bounce_rays + c.shadow_rays
+    srv.close()
+    ctx.set_counters(False)
+    for name in modes:
+        sat[name]["mrays_s"] = round(rays / sat[name]["makespan_s"] / 1e6, 1)
+    for d in resident:
+        d.destroy()
+    return {"scenes": n_scenes, "requests": n_requests, "samples": paths, "rays": int(rays),
+            "resolutions": ["256x256", "1280x720", "1920x1080"], "spp": "1-8",
+            "saturation": sat, "poisson_rate_rps": round(rate, 2), "poisson": poisson,
+            "served_vs_plain_throughput": round(sat["served"]["throughput_rps"] / sat["plain"]["throughput_rps"], 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +149,8 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity check")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream benchmark")
+    ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one elementwise kernel that reads and writes BYTES (counter calibration)")
     args = ap.parse_args()
@@ -150,6 +244,11 @@ def main():
               "traversal": {"nodes_per_closest_ray": round(tot["node_visits"] / max(1, tot["primary_rays"] +
                                                                                     tot["bounce_rays"]), 2),
                             "nodes_per_shadow_ray": round(tot["occ_node_visits"] / max(1, tot["shadow_rays"]), 2)}}
+
+    if not args.no_serve:
+        t_s = time.perf_counter()
+        result["serving"] = serving_benchmark(ctx, n_requests=args.serve_requests)
+        log(f"serving benchmark {time.perf_counter() - t_s:.1f}s")
 
     if not args.no_cpu:
         # GPU frame 0 with primary hits, for the full-size parity check

@@ -356,6 +356,77 @@ int rtg_get_counters(rtg_ctx* ctx, rtg_trace_counters* out);
 int rtg_set_timing(rtg_ctx* ctx, int enable);
 int rtg_get_kernel_time(rtg_ctx* ctx, uint32_t kind, double* total_ms, uint64_t* launches);
 
+/* ------------------------------------------------------------------------ */
+/* Serving: render requests over resident scenes with continuous batching.  */
+/* No counterpart in the reference, which renders one frame at a time       */
+/* (src/main.rs:916-996); this is the request loop a renderer service puts  */
+/* around the trace step. A worker thread forms batches from pending        */
+/* requests whenever a slot (a stream with its own frame state) is free and */
+/* the batch fits the slot's share of the HBM budget; batches on different  */
+/* slots overlap on the device. Results do not depend on the batch a        */
+/* request lands in. While a server runs, the context must not be used from */
+/* other threads except for rtg_server_* calls.                             */
+/* ------------------------------------------------------------------------ */
+typedef struct rtg_server rtg_server;
+
+typedef struct rtg_server_config {
+    uint64_t hbm_budget_bytes; /* frame state + TLAS copies of all in-flight batches */
+    uint32_t max_batch_jobs;   /* requests per batch; 1 = one request per launch */
+    uint32_t max_inflight;     /* batches in flight at once (1..8) */
+} rtg_server_config;
+
+typedef struct rtg_request {
+    uint32_t scene;                 /* id from rtg_server_add_scene */
+    uint32_t width;
+    uint32_t height;
+    rtg_trace_opts opts;
+    rtg_uniforms uniforms;
+    const rtg_instance* instances;  /* device array with this request's transforms (a private TLAS copy is
+                                       refitted with it, ≙ write_resources, src/scene.rs:167-204); NULL = as built */
+    uint32_t* out_rgba8;            /* device outputs as in rtg_image_job */
+    float* out_rgb;
+    rtg_hit* out_primary_hits;
+    uint64_t tag;                   /* returned in the completion */
+} rtg_request;
+
+typedef struct rtg_completion {
+    uint64_t ticket;
+    uint64_t tag;
+    int32_t status;    /* RTG_OK or the launch error */
+    uint32_t batch_size;
+    double submit_s;   /* server clock (seconds since rtg_server_create) */
+    double start_s;    /* its batch was launched */
+    double done_s;     /* the worker saw the batch finish */
+} rtg_completion;
+
+typedef struct rtg_server_stats {
+    uint64_t requests;
+    uint64_t batches;
+    uint64_t rejected;
+    uint64_t max_batch;
+    uint64_t max_inflight_seen;
+    uint64_t peak_bytes;
+} rtg_server_stats;
+
+int rtg_server_create(rtg_ctx* ctx, const rtg_server_config* cfg, rtg_server** out);
+/* Registers a resident scene: its TLAS (the template copied per request) and
+ * its bindings. Scenes stay resident until the server is destroyed. */
+int rtg_server_add_scene(rtg_server* srv, rtg_tlas tlas, const rtg_scene_bindings* scene, uint32_t* out_id);
+/* Queues a request (never blocks on the device). RTG_ERROR_BUDGET if it can
+ * never fit one slot of the budget. */
+int rtg_server_submit(rtg_server* srv, const rtg_request* req, uint64_t* out_ticket);
+/* Copies up to `capacity` completions; with wait_all, first waits until every
+ * submitted request has completed. */
+int rtg_server_poll(rtg_server* srv, rtg_completion* out, uint32_t capacity, uint32_t* count, int wait_all);
+/* Submits requests[i] at arrival_s[i] seconds after the call (a native
+ * arrival clock), waits for all of them and writes their completions in
+ * request order. */
+int rtg_server_replay(rtg_server* srv, const rtg_request* requests, const double* arrival_s, uint32_t count,
+                      rtg_completion* out);
+double rtg_server_clock(const rtg_server* srv);
+int rtg_server_get_stats(const rtg_server* srv, rtg_server_stats* out);
+int rtg_server_destroy(rtg_server* srv);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -725,6 +725,14 @@ int refresh_registry(rtg_ctx* ctx) {
     return RTG_OK;
 }
 
+int launch_refit(rtg_ctx* ctx, uint8_t* tlas, const rtg_instance* instances, uint32_t count, hipStream_t s) {
+    LaunchTimer timer(ctx, RTG_KERNEL_REFIT, s);
+    hipLaunchKernelGGL(k_tlas_refit, dim3(1), dim3(1024), 0, s, tlas, instances, count, ctx->d_registry,
+                       ctx->registry_count, ctx->d_status);
+    RTG_HIP(hipGetLastError());
+    return RTG_OK;
+}
+
 int check_status(rtg_ctx* ctx, hipStream_t stream, uint32_t mask, const char* what) {
     uint32_t st = 0;
     RTG_HIP(hipStreamSynchronize(stream));
@@ -905,12 +913,7 @@ extern "C" int rtg_tlas_refit(rtg_ctx* ctx, rtg_tlas tlas, const rtg_instance* i
                 "rtg_tlas_refit: 0x%llx is not a TLAS of this context", (unsigned long long)tlas);
     RTG_REQUIRE(it->second.prim_count == count, RTG_ERROR_INVALID_ARGUMENT,
                 "rtg_tlas_refit: %u instances, the TLAS was built with %u", count, it->second.prim_count);
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    LaunchTimer timer(ctx, RTG_KERNEL_REFIT, s);
-    hipLaunchKernelGGL(k_tlas_refit, dim3(1), dim3(1024), 0, s, reinterpret_cast<uint8_t*>(tlas), instances, count,
-                       ctx->d_registry, ctx->registry_count, ctx->d_status);
-    RTG_HIP(hipGetLastError());
-    return RTG_OK;
+    return launch_refit(ctx, reinterpret_cast<uint8_t*>(tlas), instances, count, reinterpret_cast<hipStream_t>(stream));
 }
 
 extern "C" int rtg_tlas_clone(rtg_ctx* ctx, rtg_tlas src, rtg_tlas* out, void* stream) {

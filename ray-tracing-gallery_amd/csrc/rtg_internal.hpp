@@ -131,4 +131,16 @@ struct LaunchTimer {
 };
 int check_status(rtg_ctx* ctx, hipStream_t stream, uint32_t mask, const char* what);
 void free_frame_arena(rtg_ctx* ctx);
+FrameArena* new_frame_arena();
+void delete_frame_arena(FrameArena* a);
+size_t frame_bytes(uint64_t paths, uint64_t shadow, uint32_t njobs);
+// The trace step of a batch of jobs on `stream` with the frame state in `a`
+// (the context's own arena if null); check_tlas validates each job's TLAS
+// against the context's structures.
+int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStream_t stream, FrameArena* a,
+               bool check_tlas);
+// Launches made by the calling thread are not timed (a server's worker).
+void suppress_timing_on_this_thread();
+// One-kernel refit of the TLAS blob at `tlas` (no registry lookup).
+int launch_refit(rtg_ctx* ctx, uint8_t* tlas, const rtg_instance* instances, uint32_t count, hipStream_t stream);
 }  // namespace rtg

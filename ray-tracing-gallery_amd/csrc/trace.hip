@@ -780,16 +780,38 @@ struct FrameArena {
     uint32_t* d_counts = nullptr;
 };
 
-void free_frame_arena(rtg_ctx* ctx) {
-    if (!ctx->frame) return;
-    FrameArena* a = ctx->frame;
+FrameArena* new_frame_arena() { return new FrameArena(); }
+
+void delete_frame_arena(FrameArena* a) {
+    if (!a) return;
     a->buf.release();
     a->spill.release();
     a->feed.release();
+    if (a->staging_free) hipEventSynchronize(a->staging_free);
     if (a->h_jobs) hipHostFree(a->h_jobs);
     if (a->staging_free) hipEventDestroy(a->staging_free);
     delete a;
+}
+
+void free_frame_arena(rtg_ctx* ctx) {
+    delete_frame_arena(ctx->frame);
     ctx->frame = nullptr;
+}
+
+// Device bytes of the frame state of `paths` paths and `shadow` shadow rays.
+size_t frame_bytes(uint64_t paths, uint64_t shadow, uint32_t njobs) {
+    size_t off = 0;
+    carve_size<float4>(off, paths * 3);
+    carve_size<uint32_t>(off, paths * 2);
+    carve_size<float4>(off, paths * 4);
+    carve_size<uint32_t>(off, paths * 2);
+    carve_size<uint4>(off, paths);
+    carve_size<float2>(off, paths);
+    carve_size<float4>(off, shadow * 2);
+    carve_size<uint32_t>(off, shadow);
+    carve_size<uint32_t>(off, kCountSlots);
+    carve_size<JobDev>(off, njobs);
+    return off + 16 * 256 + 4096;
 }
 
 static int trace_grid(rtg_ctx* ctx) {
@@ -809,8 +831,7 @@ static FrameArena* arena(rtg_ctx* ctx) {
     return ctx->frame;
 }
 
-static int ensure_frame(rtg_ctx* ctx, uint64_t paths, uint64_t shadow, uint32_t njobs) {
-    FrameArena* a = arena(ctx);
+static int ensure_frame(rtg_ctx* ctx, FrameArena* a, uint64_t paths, uint64_t shadow, uint32_t njobs) {
     if (paths > a->paths_cap || shadow > a->shadow_cap || njobs > a->jobs_cap || a->buf.ptr == nullptr) {
         uint64_t P = std::max<uint64_t>(paths, a->paths_cap);
         uint64_t S = std::max<uint64_t>(shadow, a->shadow_cap);
@@ -868,10 +889,12 @@ static int ensure_frame(rtg_ctx* ctx, uint64_t paths, uint64_t shadow, uint32_t 
     return RTG_OK;
 }
 
-static int validate_job(rtg_ctx* ctx, const rtg_image_job& j, uint32_t k) {
-    auto it = ctx->structures.find(j.tlas);
-    RTG_REQUIRE(it != ctx->structures.end() && it->second.kind == kKindTlas, RTG_ERROR_NOT_FOUND,
-                "job %u: 0x%llx is not a TLAS of this context", k, (unsigned long long)j.tlas);
+static int validate_job(rtg_ctx* ctx, const rtg_image_job& j, uint32_t k, bool check_tlas) {
+    if (check_tlas) {
+        auto it = ctx->structures.find(j.tlas);
+        RTG_REQUIRE(it != ctx->structures.end() && it->second.kind == kKindTlas, RTG_ERROR_NOT_FOUND,
+                    "job %u: 0x%llx is not a TLAS of this context", k, (unsigned long long)j.tlas);
+    }
     RTG_REQUIRE(j.width > 0 && j.height > 0, RTG_ERROR_INVALID_ARGUMENT, "job %u: empty image %ux%u", k, j.width,
                 j.height);
     RTG_REQUIRE(j.opts.spp >= 1 && j.opts.spp <= 64, RTG_ERROR_INVALID_ARGUMENT, "job %u: spp %u (1..64)", k,
@@ -887,12 +910,14 @@ static int validate_job(rtg_ctx* ctx, const rtg_image_job& j, uint32_t k) {
     return RTG_OK;
 }
 
-int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStream_t s) {
+int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStream_t s, FrameArena* a,
+               bool check_tlas) {
     RTG_REQUIRE(njobs > 0 && jobs, RTG_ERROR_INVALID_ARGUMENT, "trace: no jobs");
+    if (a == nullptr) a = arena(ctx);
     uint64_t paths = 0, pixels = 0, shadow = 0;
     uint32_t max_traces = 0;
     for (uint32_t k = 0; k < njobs; ++k) {
-        int st = validate_job(ctx, jobs[k], k);
+        int st = validate_job(ctx, jobs[k], k, check_tlas);
         if (st != RTG_OK) return st;
         uint64_t px = (uint64_t)jobs[k].width * jobs[k].height;
         paths += px * jobs[k].opts.spp;
@@ -904,9 +929,8 @@ int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStrea
                 "trace: %llu paths / %llu shadow rays exceed one launch", (unsigned long long)paths,
                 (unsigned long long)shadow);
     RTG_HIP(hipSetDevice(ctx->device));
-    int st = ensure_frame(ctx, paths, shadow, njobs);
+    int st = ensure_frame(ctx, a, paths, shadow, njobs);
     if (st != RTG_OK) return st;
-    FrameArena* a = ctx->frame;
     // job table through pinned staging; wait until the previous upload left it
     RTG_HIP(hipEventSynchronize(a->staging_free));
     uint64_t sb = 0, pb = 0;
@@ -1020,14 +1044,14 @@ using namespace rtg;
 extern "C" int rtg_trace_image(rtg_ctx* ctx, const rtg_image_job* job, void* stream) {
     clear_error();
     RTG_REQUIRE(ctx && job, RTG_ERROR_INVALID_ARGUMENT, "rtg_trace_image: null argument");
-    return trace_jobs(ctx, job, 1, reinterpret_cast<hipStream_t>(stream));
+    return trace_jobs(ctx, job, 1, reinterpret_cast<hipStream_t>(stream), nullptr, true);
 }
 
 extern "C" int rtg_trace_batch(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t n, void* stream) {
     clear_error();
     RTG_REQUIRE(ctx && (jobs || n == 0), RTG_ERROR_INVALID_ARGUMENT, "rtg_trace_batch: null argument");
     if (n == 0) return RTG_OK;
-    return trace_jobs(ctx, jobs, n, reinterpret_cast<hipStream_t>(stream));
+    return trace_jobs(ctx, jobs, n, reinterpret_cast<hipStream_t>(stream), nullptr, true);
 }
 
 extern "C" int rtg_trace_rays(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindings* scene, const rtg_ray* rays,

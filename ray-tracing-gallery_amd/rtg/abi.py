@@ -245,3 +245,45 @@ def uniforms_from(view_inverse, proj_inverse, sun_dir, sun_radius, blue_noise_in
     u.frame_index = frame_index
     u.show_heatmap = 1 if show_heatmap else 0
     return u
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_jobs", C.c_uint32), ("max_inflight", C.c_uint32)]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("scene", C.c_uint32),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("opts", TraceOpts),
+        ("uniforms", Uniforms),
+        ("instances", C.c_uint64),
+        ("out_rgba8", C.c_uint64),
+        ("out_rgb", C.c_uint64),
+        ("out_primary_hits", C.c_uint64),
+        ("tag", C.c_uint64),
+    ]
+
+
+class Completion(C.Structure):
+    _fields_ = [
+        ("ticket", C.c_uint64),
+        ("tag", C.c_uint64),
+        ("status", C.c_int32),
+        ("batch_size", C.c_uint32),
+        ("submit_s", C.c_double),
+        ("start_s", C.c_double),
+        ("done_s", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("batches", C.c_uint64), ("rejected", C.c_uint64),
+                ("max_batch", C.c_uint64), ("max_inflight_seen", C.c_uint64), ("peak_bytes", C.c_uint64)]
+
+    def as_dict(self):
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+
+
+RTG_ERROR_BUDGET = -6

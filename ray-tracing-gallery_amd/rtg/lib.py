@@ -18,6 +18,8 @@ EXPORTS = [
     "rtg_tlas_query_sizes", "rtg_tlas_build", "rtg_tlas_refit", "rtg_tlas_clone", "rtg_tlas_destroy",
     "rtg_as_get_info", "rtg_trace_rays", "rtg_trace_image", "rtg_trace_batch",
     "rtg_set_counters", "rtg_get_counters", "rtg_set_timing", "rtg_get_kernel_time",
+    "rtg_server_create", "rtg_server_add_scene", "rtg_server_submit", "rtg_server_poll", "rtg_server_replay",
+    "rtg_server_clock", "rtg_server_get_stats", "rtg_server_destroy",
 ]
 
 _lib = None
@@ -57,9 +59,19 @@ def load():
     L.rtg_get_counters.argtypes = [vp, C.POINTER(abi.TraceCounters)]
     L.rtg_set_timing.argtypes = [vp, i32]
     L.rtg_get_kernel_time.argtypes = [vp, u32, C.POINTER(C.c_double), C.POINTER(u64)]
+    L.rtg_server_create.argtypes = [vp, C.POINTER(abi.ServerConfig), C.POINTER(vp)]
+    L.rtg_server_add_scene.argtypes = [vp, u64, C.POINTER(abi.SceneBindings), C.POINTER(u32)]
+    L.rtg_server_submit.argtypes = [vp, C.POINTER(abi.Request), C.POINTER(u64)]
+    L.rtg_server_poll.argtypes = [vp, C.POINTER(abi.Completion), u32, C.POINTER(u32), i32]
+    L.rtg_server_replay.argtypes = [vp, C.POINTER(abi.Request), C.POINTER(C.c_double), u32,
+                                    C.POINTER(abi.Completion)]
+    L.rtg_server_clock.argtypes = [vp]
+    L.rtg_server_get_stats.argtypes = [vp, C.POINTER(abi.ServerStats)]
+    L.rtg_server_destroy.argtypes = [vp]
     for name in EXPORTS:
         if name not in ("rtg_destroy", "rtg_last_error", "rtg_abi_version"):
             getattr(L, name).restype = C.c_int
+    L.rtg_server_clock.restype = C.c_double
     _lib = L
     return L
 

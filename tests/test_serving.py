@@ -1,0 +1,109 @@
+"""Serving (continuous batching over resident scenes): every result is the
+same alone and in the mix, batches really form, the HBM budget rejects what
+can never fit."""
+import numpy as np
+import pytest
+import torch
+
+from rtg import abi, scenes
+from rtg.device import DeviceScene
+from rtg.serve import Server, make_request
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def resident(ctx):
+    out = []
+    for k in range(3):
+        s = scenes.small_scene(seed=0x5EED0000 + 100 + k, width=64, height=64, n_tori=6)
+        out.append(DeviceScene(ctx, s))
+    return out
+
+
+def request_set(resident, n=10, seed=3):
+    rng = scenes.Rng(seed)
+    sizes = [(48, 32), (64, 64), (96, 40), (33, 17)]
+    reqs, outs, bufs = [], [], []
+    for i in range(n):
+        sid = int(rng.integers(1, 0, len(resident))[0])
+        ds = resident[sid]
+        w, h = sizes[int(rng.integers(1, 0, len(sizes))[0])]
+        spp = int(rng.integers(1, 1, 4)[0])
+        # a random transform update of every torus
+        xf = [ins.transform.copy() for ins in ds.scene.instances]
+        for k in range(1, len(xf)):
+            xf[k] = scenes.affine(xf[k][:, 3], scenes.rotation_from_uniform(*rng.uniform(3)), 1.0)
+        buf = ds.instance_buffer(xf)
+        o = (torch.zeros(w * h, dtype=torch.int32, device="cuda:0"),
+             torch.zeros(w * h * 3, dtype=torch.float32, device="cuda:0"),
+             torch.zeros(w * h * spp * 6, dtype=torch.int32, device="cuda:0"))
+        r = make_request(sid, ds, w, h, spp, 2, 3, frame=i, instances=buf, outputs=o, tag=i)
+        reqs.append(r)
+        outs.append(o)
+        bufs.append((xf, buf))
+    return reqs, outs, bufs
+
+
+def snapshot(outs):
+    torch.cuda.synchronize()
+    return [tuple(t.cpu().numpy().copy() for t in o) for o in outs]
+
+
+def test_results_alone_equal_results_in_the_mix(ctx, resident):
+    reqs, outs, bufs = request_set(resident)
+    # alone: one request per launch, one launch at a time, arriving far apart
+    srv = Server(ctx, budget_bytes=1 << 30, max_batch=1, max_inflight=1)
+    ids = [srv.add_scene(ds) for ds in resident]
+    assert ids == [0, 1, 2]
+    comp = srv.replay(reqs, np.arange(len(reqs)) * 0.02)
+    assert all(c.status == 0 and c.batch_size == 1 for c in comp)
+    srv.close()
+    alone = snapshot(outs)
+    for o in outs:
+        for t in o:
+            t.zero_()
+    # in the mix: all arrive at once, batched and overlapping
+    srv = Server(ctx, budget_bytes=1 << 30, max_batch=8, max_inflight=2)
+    for ds in resident:
+        srv.add_scene(ds)
+    comp = srv.replay(reqs, np.zeros(len(reqs)))
+    stats = srv.stats()
+    srv.close()
+    assert all(c.status == 0 for c in comp)
+    assert stats["max_batch"] > 1 and stats["batches"] < len(reqs)
+    mixed = snapshot(outs)
+    for a, b in zip(alone, mixed):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+    # and equal to the plain path: refit a clone, trace one image
+    for r, (xf, buf), a in zip(reqs, bufs, alone):
+        ds = resident[r.scene]
+        clone = ds.clone_tlas()
+        ds.refit(buf, tlas=clone)
+        o = (torch.zeros(r.width * r.height, dtype=torch.int32, device="cuda:0"),
+             torch.zeros(r.width * r.height * 3, dtype=torch.float32, device="cuda:0"),
+             torch.zeros(r.width * r.height * r.opts.spp * 6, dtype=torch.int32, device="cuda:0"))
+        j = ds.job(o, tlas=clone, width=r.width, height=r.height, uniforms=r.uniforms, opts=r.opts)
+        from rtg.lib import check, load
+        import ctypes as C
+        check(load().rtg_trace_image(ctx.ptr, C.byref(j), None), "rtg_trace_image")
+        b = snapshot([o])[0]
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+        load().rtg_tlas_destroy(ctx.ptr, clone)
+
+
+def test_budget_rejects_requests_that_never_fit(ctx, resident):
+    srv = Server(ctx, budget_bytes=64 << 20, max_batch=4, max_inflight=2)
+    srv.add_scene(resident[0])
+    big = make_request(0, resident[0], 1920, 1080, 8, 4, 3, frame=0)
+    st, _ = srv.submit(big)
+    assert st == abi.RTG_ERROR_BUDGET
+    assert srv.stats()["rejected"] == 1
+    small = make_request(0, resident[0], 32, 32, 1, 1, 3, frame=0)
+    st, _ = srv.submit(small)
+    assert st == 0
+    done = srv.poll(wait_all=True)
+    assert len(done) == 1 and done[0].status == 0
+    srv.close()
