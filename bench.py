@@ -141,6 +141,161 @@ def serving_benchmark(ctx, n_scenes=32, n_requests=160, seed=0x5EED0004):
             "served_vs_plain_throughput": round(sat["served"]["throughput_rps"] / sat["plain"]["throughput_rps"], 3)}
 
 
+def batched_benchmark(ctx, steps, seed=0x5EED0003):
+    """Batched launch: 64 independent scenes (triangle counts log-uniform in
+    [1k, 500k], resolutions from {128^2, 512^2, 1280x720, 1920x1080}, 1-4
+    spp, 4 soft-shadow rays per hit) traced by one graph launch with the
+    autotuner's launch setting. Also times the same jobs through the plain
+    single-image path one after another, and as one eager batch."""
+    import numpy as np
+    import torch
+    from rtg import abi, scenes
+    from rtg.batch import Batch, autotune, trace_batch
+    from rtg.device import DeviceScene
+    import ctypes as C
+    from rtg.lib import check, load
+
+    t0 = time.perf_counter()
+    ws = scenes.batch_workload(seed=seed)
+    t_gen = time.perf_counter() - t0
+    dss = [DeviceScene(ctx, s) for s in ws]
+    outs = [ds.make_outputs(hits=False) for ds in dss]
+    jobs = [ds.job(o, frame_index=0) for ds, o in zip(dss, outs)]
+    torch.cuda.synchronize()
+    samples = sum(s.width * s.height * s.spp for s in ws)
+    tris = sum(s.instanced_triangles for s in ws)
+    log(f"batched setup {time.perf_counter() - t0:.1f}s (generation {t_gen:.1f}s): {len(ws)} scenes, "
+        f"{tris} instanced triangles, {samples / 1e6:.1f} M samples")
+
+    def wall(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            a = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - a)
+        return float(np.median(ts))
+
+    def plain():
+        for j in jobs:
+            check(load().rtg_trace_image(ctx.ptr, C.byref(j), None), "rtg_trace_image")
+
+    default = ctx.tuning()
+    t_plain = wall(plain, 1)
+    t_eager = wall(lambda: trace_batch(ctx, jobs), 2)
+    a = time.perf_counter()
+    best, tuned_ms = autotune(ctx, jobs, repeats=1)
+    t_tune = time.perf_counter() - a
+    log(f"autotune {t_tune:.1f}s: {best.as_dict()} {tuned_ms:.1f} ms")
+    b = Batch(ctx, jobs)
+    b.launch()
+    torch.cuda.synchronize()
+    ctx.set_timing(True)
+    a = time.perf_counter()
+    for _ in range(steps):
+        b.launch()
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - a) / steps
+    g_ms, g_n = ctx.kernel_time(abi.RTG_KERNEL_BATCH_GRAPH)
+    ctx.set_timing(False)
+    b.close()
+    # exact ray count: a counting replay of the same batch
+    ctx.set_counters(True)
+    bc = Batch(ctx, jobs)
+    bc.launch()
+    c = ctx.counters()
+    bc.close()
+    ctx.set_counters(False)
+    rays = c.primary_rays + c.bounce_rays + c.shadow_rays
+    ctx.set_tuning(default)
+    for d in dss:
+        d.destroy()
+    del outs
+    torch.cuda.empty_cache()
+    return {"scenes": len(ws), "instanced_triangles": int(tris), "samples": int(samples), "rays": int(rays),
+            "tri_range": [1000, 500000], "resolutions": ["128x128", "512x512", "1280x720", "1920x1080"],
+            "spp": "1-4", "shadow_rays": 4, "seed": seed,
+            "tuning": best.as_dict(), "autotune_s": round(t_tune, 1),
+            "graph": {"ms_per_batch": round(g_ms / max(1, g_n), 3), "wall_ms_per_batch": round(t_graph * 1e3, 3),
+                      "mrays_s": round(rays / (g_ms / max(1, g_n) / 1e3) / 1e6, 1), "launches": int(g_n)},
+            "eager_batch": {"wall_ms": round(t_eager * 1e3, 3), "mrays_s": round(rays / t_eager / 1e6, 1)},
+            "plain_one_by_one": {"wall_ms": round(t_plain * 1e3, 3), "mrays_s": round(rays / t_plain / 1e6, 1)},
+            "graph_vs_plain": round(t_plain / t_graph, 3)}
+
+
+def incoherent_benchmark(ctx, n_rays=16 << 20, sample=65536):
+    """Incoherent-ray stress: one BLAS of 1,000,000 random triangles
+    (lognormal edge lengths, median 0.01) in the unit cube; 16M rays with
+    origins uniform in the cube and directions uniform on the sphere, tmin 0,
+    tmax +inf; closest-hit and occlusion queries. Reports Mrays/s, node
+    visits per ray and algorithmic GB/s, and checks a ray sample bit for bit
+    against the oracle (also timed there, on the host cores)."""
+    import numpy as np
+    import torch
+    from rtg import abi, scenes
+    from rtg.device import DeviceScene, hits_to_numpy, rays_to_device
+
+    t0 = time.perf_counter()
+    scene = scenes.incoherent_scene()
+    rays = scenes.incoherent_rays(n_rays)
+    ds = DeviceScene(ctx, scene)
+    rd = rays_to_device(rays)
+    hits = torch.empty(n_rays * 6, dtype=torch.int32, device="cuda:0")
+    occ = torch.empty(n_rays, dtype=torch.int32, device="cuda:0")
+    torch.cuda.synchronize()
+    log(f"incoherent setup {time.perf_counter() - t0:.1f}s")
+    out = {"triangles": scene.instanced_triangles, "rays": n_rays, "seed": scene.seed}
+    for name, mode, buf, kind in (("closest", abi.RTG_QUERY_CLOSEST, hits, abi.RTG_KERNEL_TRACE_CLOSEST),
+                                  ("occlusion", abi.RTG_QUERY_OCCLUSION, occ, abi.RTG_KERNEL_TRACE_OCCLUSION)):
+        ds.trace_rays(rd, n_rays, mode, out=buf)
+        torch.cuda.synchronize()
+        ctx.set_timing(True)
+        for _ in range(3):
+            ds.trace_rays(rd, n_rays, mode, out=buf)
+        torch.cuda.synchronize()
+        ms, n = ctx.kernel_time(kind)
+        ctx.set_timing(False)
+        ms /= max(1, n)
+        ctx.set_counters(True)
+        ds.trace_rays(rd, n_rays, mode, out=buf)
+        c = ctx.counters()
+        ctx.set_counters(False)
+        if name == "closest":
+            nb = n_rays * (32 + 24) + c.node_visits * 64 + c.triangle_tests * 48 + c.instance_visits * 112
+            nodes, trisv = c.node_visits, c.triangle_tests
+        else:
+            nb = n_rays * (32 + 4) + c.occ_node_visits * 64 + c.occ_triangle_tests * 48 + c.occ_instance_visits * 112
+            nodes, trisv = c.occ_node_visits, c.occ_triangle_tests
+        out[name] = {"ms": round(ms, 3), "mrays_s": round(n_rays / (ms / 1e3) / 1e6, 1),
+                     "nodes_per_ray": round(nodes / n_rays, 2), "tris_per_ray": round(trisv / n_rays, 2),
+                     "algorithmic_gb_s": round(nb / (ms / 1e3) / 1e9, 1)}
+    torch.cuda.synchronize()
+    # bit-exact check of a ray sample against the oracle (same BVH, same order)
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_host as oh
+    idx = np.linspace(0, n_rays - 1, sample).astype(np.int64)
+    hs = oh.HostScene(scene)
+    threads = len(os.sched_getaffinity(0))
+    a = time.perf_counter()
+    o_hits, _ = hs.trace_rays(rays[idx], abi.RTG_QUERY_CLOSEST, threads=threads)
+    o_occ, _ = hs.trace_rays(rays[idx], abi.RTG_QUERY_OCCLUSION, threads=threads)
+    cpu_s = time.perf_counter() - a
+    g_hits = hits_to_numpy(hits)[idx]
+    g_occ = occ.cpu().numpy()[idx]
+    out["parity_sample"] = {"rays": sample,
+                            "hits_bit_exact": bool(np.array_equal(g_hits.view(np.uint32), o_hits.view(np.uint32))),
+                            "occlusion_exact": bool(np.array_equal(g_occ.astype(np.uint32),
+                                                                   np.asarray(o_occ).astype(np.uint32)))}
+    out["cpu_oracle_mrays_s"] = round(2 * sample / cpu_s / 1e6, 3)
+    out["cpu_cores"] = threads
+    ds.destroy()
+    del hits, occ, rd
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +306,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity check")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream benchmark")
     ap.add_argument("--serve-requests", type=int, default=160)
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene graph-batch benchmark")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-incoherent", action="store_true", help="skip the 16M incoherent-ray benchmark")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one elementwise kernel that reads and writes BYTES (counter calibration)")
     args = ap.parse_args()
@@ -249,6 +407,16 @@ def main():
         t_s = time.perf_counter()
         result["serving"] = serving_benchmark(ctx, n_requests=args.serve_requests)
         log(f"serving benchmark {time.perf_counter() - t_s:.1f}s")
+
+    if not args.no_batch:
+        t_b = time.perf_counter()
+        result["batched"] = batched_benchmark(ctx, args.batch_steps)
+        log(f"batched benchmark {time.perf_counter() - t_b:.1f}s")
+
+    if not args.no_incoherent:
+        t_i = time.perf_counter()
+        result["incoherent"] = incoherent_benchmark(ctx)
+        log(f"incoherent benchmark {time.perf_counter() - t_i:.1f}s")
 
     if not args.no_cpu:
         # GPU frame 0 with primary hits, for the full-size parity check

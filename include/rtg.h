@@ -352,9 +352,53 @@ int rtg_get_counters(rtg_ctx* ctx, rtg_trace_counters* out);
 #define RTG_KERNEL_TRACE_OCCLUSION 3u
 #define RTG_KERNEL_RESOLVE 4u
 #define RTG_KERNEL_REFIT 5u
-#define RTG_KERNEL_COUNT 6u
+#define RTG_KERNEL_BATCH_GRAPH 6u /* one replay of an rtg_batch graph (all its kernels) */
+#define RTG_KERNEL_COUNT 7u
 int rtg_set_timing(rtg_ctx* ctx, int enable);
 int rtg_get_kernel_time(rtg_ctx* ctx, uint32_t kind, double* total_ms, uint64_t* launches);
+
+/* ------------------------------------------------------------------------ */
+/* Launch tuning and the autotuner.                                         */
+/* No counterpart in the reference (the Vulkan driver picks its own launch  */
+/* shape for vkCmdTraceRaysKHR, command_buffer_recording.rs:116-126). These */
+/* parameters change speed only: every setting gives identical results.     */
+/* ------------------------------------------------------------------------ */
+typedef struct rtg_tuning {
+    uint32_t blocks_per_cu;    /* persistent traversal workgroups (256 lanes) per CU, 1..8 */
+    uint32_t feed_chunk;       /* queue entries a wavefront takes per atomic, 64..1024, multiple of 64 */
+    uint32_t refill_closest;   /* idle lanes (1..64) before a wavefront takes new closest-hit rays;
+                                  0 = non-persistent kernel (grid-stride rounds) */
+    uint32_t refill_occlusion; /* the same for occlusion rays */
+    uint32_t path_order;       /* 0 = row-major paths, 1 = 8x8 pixel blocks per wavefront */
+    uint32_t reserved[3];      /* zero */
+} rtg_tuning;
+int rtg_get_tuning(rtg_ctx* ctx, rtg_tuning* out);
+int rtg_set_tuning(rtg_ctx* ctx, const rtg_tuning* tuning);
+/* Times the jobs (as one rtg_trace_batch, `repeats` timed runs per setting
+ * after one warm-up) over a coordinate search of the tuning space starting
+ * from the current setting, makes the fastest setting current and returns it
+ * with its median time in ms. Writes the jobs' outputs (synchronises). */
+int rtg_autotune(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t job_count, uint32_t repeats, rtg_tuning* best,
+                 double* best_ms);
+
+/* ------------------------------------------------------------------------ */
+/* Graph-captured batches.                                                  */
+/* No counterpart in the reference (one vkCmdTraceRaysKHR per frame,        */
+/* command_buffer_recording.rs:116-126). A batch is a fixed set of jobs     */
+/* (scenes and images of uneven size) whose whole trace step -- ray         */
+/* generation, persistent traversal, shading, occlusion, resolve -- is      */
+/* captured once into a HIP graph with the context's tuning and counter     */
+/* setting of that moment and replayed by one graph launch. The batch owns  */
+/* its frame state; it does not share it with other calls.                  */
+/* ------------------------------------------------------------------------ */
+typedef struct rtg_batch rtg_batch;
+int rtg_batch_create(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t job_count, rtg_batch** out);
+/* Replaces the job table (TLAS, bindings, uniforms, outputs) for the next
+ * launches; width, height and opts of every job must be unchanged. The copy
+ * is ordered on `stream`. */
+int rtg_batch_update(rtg_batch* batch, const rtg_image_job* jobs, uint32_t job_count, void* stream);
+int rtg_batch_launch(rtg_batch* batch, void* stream);
+int rtg_batch_destroy(rtg_batch* batch);
 
 /* ------------------------------------------------------------------------ */
 /* Serving: render requests over resident scenes with continuous batching.  */
@@ -445,6 +489,7 @@ static_assert(sizeof(rtg_geometry_info) == 24, "GeometryInfo is 24 bytes");
 static_assert(sizeof(rtg_ray) == 32, "ray is 32 bytes");
 static_assert(sizeof(rtg_hit) == 24, "hit is 24 bytes");
 static_assert(sizeof(rtg_texture) == 24, "texture descriptor is 24 bytes");
+static_assert(sizeof(rtg_tuning) == 32, "tuning is 32 bytes");
 #else
 _Static_assert(sizeof(rtg_instance) == 64, "instance record is 64 bytes");
 _Static_assert(sizeof(rtg_uniforms) == 176, "Uniforms is 176 bytes");

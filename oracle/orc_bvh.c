@@ -388,7 +388,7 @@ static void fill_inst_record(orc_inst* rec, const rtg_instance* in, uint32_t ind
 
 int orc_tlas_build(const rtg_instance* inst, uint32_t count, void** out_blob, uint64_t* out_size) {
     if (count >= (1u << 27)) return RTG_ERROR_LIMIT;
-    box3* boxes = (box3*)malloc(sizeof(box3) * (count ? count : 1));
+    box3* boxes = (box3*)calloc(count ? count : 1, sizeof(box3));
     for (uint32_t i = 0; i < count; ++i) boxes[i] = instance_box(&inst[i]);
     lbvh b;
     int st = lbvh_build(boxes, count, &b);

@@ -52,8 +52,8 @@ void DeviceBuffer::release() {
 static thread_local bool t_no_timing = false;
 void suppress_timing_on_this_thread() { t_no_timing = true; }
 
-LaunchTimer::LaunchTimer(rtg_ctx* c, uint32_t k, hipStream_t s) : ctx(c), kind(k), stream(s) {
-    if (!ctx->timing || t_no_timing) return;
+LaunchTimer::LaunchTimer(rtg_ctx* c, uint32_t k, hipStream_t s, bool on) : ctx(c), kind(k), stream(s) {
+    if (!on || !ctx->timing || t_no_timing) return;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
         a = b = nullptr;
         return;

@@ -126,7 +126,8 @@ struct LaunchTimer {
     uint32_t kind;
     hipStream_t stream;
     hipEvent_t a = nullptr, b = nullptr;
-    LaunchTimer(rtg_ctx* c, uint32_t k, hipStream_t s);
+    // `on` false: no events (launches being captured into a graph)
+    LaunchTimer(rtg_ctx* c, uint32_t k, hipStream_t s, bool on = true);
     ~LaunchTimer();
 };
 int check_status(rtg_ctx* ctx, hipStream_t stream, uint32_t mask, const char* what);
@@ -139,6 +140,29 @@ size_t frame_bytes(uint64_t paths, uint64_t shadow, uint32_t njobs);
 // against the context's structures.
 int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStream_t stream, FrameArena* a,
                bool check_tlas);
+// A trace step split in its host half and its device half.
+struct TracePlan {
+    uint64_t paths = 0, pixels = 0, shadow = 0;
+    uint32_t njobs = 0, max_traces = 0;
+};
+// Host half: validates the jobs, sizes the arena and fills its pinned job
+// table (after the previous upload from it has left).
+// The arena's spill area is sized for `tune`.
+int plan_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, FrameArena* a, bool check_tlas,
+              const Tuning& tune, TracePlan& P);
+// The frame state of the context's own calls (created on first use).
+FrameArena* context_arena(rtg_ctx* ctx);
+// Device pointers of an arena's buffers (a captured graph keeps them).
+void arena_pointers(const FrameArena* a, const void* out[3]);
+// Copies the pinned job table to the device on `stream` (asynchronous).
+int upload_jobs(FrameArena* a, const TracePlan& P, hipStream_t stream);
+// Device half: the kernels of the step and nothing else (no copy, memset or
+// allocation), so that it can be captured into a graph. Launch parameters
+// come from `tune`; `timed` records kernel events when timing is on.
+void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning& tune, bool counters, bool timed,
+                   hipStream_t stream);
+// Device bytes of the spill area of the persistent traversal grid.
+size_t spill_bytes(rtg_ctx* ctx, const Tuning& tune);
 // Launches made by the calling thread are not timed (a server's worker).
 void suppress_timing_on_this_thread();
 // One-kernel refit of the TLAS blob at `tlas` (no registry lookup).

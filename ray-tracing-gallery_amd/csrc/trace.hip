@@ -793,6 +793,12 @@ void delete_frame_arena(FrameArena* a) {
     delete a;
 }
 
+void arena_pointers(const FrameArena* a, const void* out[3]) {
+    out[0] = a->buf.ptr;
+    out[1] = a->spill.ptr;
+    out[2] = a->h_jobs;
+}
+
 void free_frame_arena(rtg_ctx* ctx) {
     delete_frame_arena(ctx->frame);
     ctx->frame = nullptr;
@@ -814,15 +820,19 @@ size_t frame_bytes(uint64_t paths, uint64_t shadow, uint32_t njobs) {
     return off + 16 * 256 + 4096;
 }
 
-static int trace_grid(rtg_ctx* ctx) {
+static int trace_grid(rtg_ctx* ctx, const Tuning& tune) {
     // persistent grid: blocks_per_cu blocks of 256 lanes per CU (the LDS
     // stack takes 32 KB per block, so at most 5 are resident)
-    return ctx->cu_count * ctx->tune.blocks_per_cu;
+    return ctx->cu_count * tune.blocks_per_cu;
+}
+static int trace_grid(rtg_ctx* ctx) { return trace_grid(ctx, ctx->tune); }
+
+size_t spill_bytes(rtg_ctx* ctx, const Tuning& tune) {
+    return (size_t)trace_grid(ctx, tune) * kTraceThreads * kSpillDepth * sizeof(uint2);
 }
 
-static int ensure_spill(rtg_ctx* ctx, FrameArena* a) {
-    size_t need = (size_t)trace_grid(ctx) * kTraceThreads * kSpillDepth * sizeof(uint2);
-    if (a->spill.ensure(need) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
+static int ensure_spill(rtg_ctx* ctx, FrameArena* a, const Tuning& tune) {
+    if (a->spill.ensure(spill_bytes(ctx, tune)) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
     return a->feed.ensure(256);
 }
 
@@ -830,8 +840,10 @@ static FrameArena* arena(rtg_ctx* ctx) {
     if (!ctx->frame) ctx->frame = new FrameArena();
     return ctx->frame;
 }
+FrameArena* context_arena(rtg_ctx* ctx) { return arena(ctx); }
 
-static int ensure_frame(rtg_ctx* ctx, FrameArena* a, uint64_t paths, uint64_t shadow, uint32_t njobs) {
+static int ensure_frame(rtg_ctx* ctx, FrameArena* a, uint64_t paths, uint64_t shadow, uint32_t njobs,
+                        const Tuning& tune) {
     if (paths > a->paths_cap || shadow > a->shadow_cap || njobs > a->jobs_cap || a->buf.ptr == nullptr) {
         uint64_t P = std::max<uint64_t>(paths, a->paths_cap);
         uint64_t S = std::max<uint64_t>(shadow, a->shadow_cap);
@@ -878,7 +890,7 @@ static int ensure_frame(rtg_ctx* ctx, FrameArena* a, uint64_t paths, uint64_t sh
         a->shadow_cap = S;
         a->jobs_cap = NJ;
     }
-    if (ensure_spill(ctx, a) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
+    if (ensure_spill(ctx, a, tune) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
     if (njobs > a->h_jobs_cap) {
         if (a->staging_free) hipEventSynchronize(a->staging_free);
         if (a->h_jobs) hipHostFree(a->h_jobs);
@@ -910,28 +922,27 @@ static int validate_job(rtg_ctx* ctx, const rtg_image_job& j, uint32_t k, bool c
     return RTG_OK;
 }
 
-int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStream_t s, FrameArena* a,
-               bool check_tlas) {
+int plan_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, FrameArena* a, bool check_tlas,
+              const Tuning& tune, TracePlan& P) {
     RTG_REQUIRE(njobs > 0 && jobs, RTG_ERROR_INVALID_ARGUMENT, "trace: no jobs");
-    if (a == nullptr) a = arena(ctx);
-    uint64_t paths = 0, pixels = 0, shadow = 0;
-    uint32_t max_traces = 0;
+    P = TracePlan();
+    P.njobs = njobs;
     for (uint32_t k = 0; k < njobs; ++k) {
         int st = validate_job(ctx, jobs[k], k, check_tlas);
         if (st != RTG_OK) return st;
         uint64_t px = (uint64_t)jobs[k].width * jobs[k].height;
-        paths += px * jobs[k].opts.spp;
-        pixels += px;
-        shadow += px * jobs[k].opts.spp * jobs[k].opts.shadow_rays;
-        max_traces = std::max(max_traces, jobs[k].opts.max_traces);
+        P.paths += px * jobs[k].opts.spp;
+        P.pixels += px;
+        P.shadow += px * jobs[k].opts.spp * jobs[k].opts.shadow_rays;
+        P.max_traces = std::max(P.max_traces, jobs[k].opts.max_traces);
     }
-    RTG_REQUIRE(paths < (1ull << 31) && shadow < (1ull << 32), RTG_ERROR_LIMIT,
-                "trace: %llu paths / %llu shadow rays exceed one launch", (unsigned long long)paths,
-                (unsigned long long)shadow);
+    RTG_REQUIRE(P.paths < (1ull << 31) && P.shadow < (1ull << 32), RTG_ERROR_LIMIT,
+                "trace: %llu paths / %llu shadow rays exceed one launch", (unsigned long long)P.paths,
+                (unsigned long long)P.shadow);
     RTG_HIP(hipSetDevice(ctx->device));
-    int st = ensure_frame(ctx, a, paths, shadow, njobs);
+    int st = ensure_frame(ctx, a, P.paths, P.shadow, njobs, tune);
     if (st != RTG_OK) return st;
-    // job table through pinned staging; wait until the previous upload left it
+    // the pinned job table is reused: wait until the previous upload left it
     RTG_HIP(hipEventSynchronize(a->staging_free));
     uint64_t sb = 0, pb = 0;
     for (uint32_t k = 0; k < njobs; ++k) {
@@ -965,33 +976,41 @@ int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStrea
         sb += px * j.opts.spp;
         pb += px;
     }
-    RTG_HIP(hipMemcpyAsync(a->d_jobs, a->h_jobs, njobs * sizeof(JobDev), hipMemcpyHostToDevice, s));
+    return RTG_OK;
+}
+
+int upload_jobs(FrameArena* a, const TracePlan& P, hipStream_t s) {
+    RTG_HIP(hipMemcpyAsync(a->d_jobs, a->h_jobs, P.njobs * sizeof(JobDev), hipMemcpyHostToDevice, s));
     RTG_HIP(hipEventRecord(a->staging_free, s));
+    return RTG_OK;
+}
+
+void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning& tune, bool cnt, bool timed,
+                   hipStream_t s) {
     Frame F = a->F;
     F.jobs = a->d_jobs;
-    F.njobs = njobs;
-    F.paths = paths;
+    F.njobs = P.njobs;
+    F.paths = P.paths;
     F.counts = a->d_counts;
     F.counters = ctx->d_counters;
     F.spill = reinterpret_cast<uint2*>(a->spill.ptr);
-    F.order = (uint32_t)ctx->tune.path_order;
-    F.refill_closest = ctx->tune.refill_closest;
-    F.refill_occlusion = ctx->tune.refill_occlusion;
-    F.feed_chunk = (uint32_t)ctx->tune.feed_chunk;
-    const bool cnt = ctx->counters_enabled;
-    const unsigned G = (unsigned)trace_grid(ctx);
-    const unsigned Gsmall = (unsigned)std::min<uint64_t>((paths + 255) / 256, (uint64_t)ctx->cu_count * 16);
+    F.order = (uint32_t)tune.path_order;
+    F.refill_closest = tune.refill_closest;
+    F.refill_occlusion = tune.refill_occlusion;
+    F.feed_chunk = (uint32_t)tune.feed_chunk;
+    const unsigned G = (unsigned)trace_grid(ctx, tune);
+    const unsigned Gsmall = (unsigned)std::min<uint64_t>((P.paths + 255) / 256, (uint64_t)ctx->cu_count * 16);
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, a->d_counts, ctx->d_counters, cnt);
     {
-        LaunchTimer t(ctx, RTG_KERNEL_RAYGEN, s);
+        LaunchTimer t(ctx, RTG_KERNEL_RAYGEN, s, timed);
         hipLaunchKernelGGL(k_raygen, dim3(Gsmall), dim3(256), 0, s, F, cnt);
     }
-    for (uint32_t level = 0; level < max_traces; ++level) {
+    for (uint32_t level = 0; level < P.max_traces; ++level) {
         const uint32_t* cp = level == 0 ? nullptr : a->d_counts + level;
-        uint32_t fixed = level == 0 ? (uint32_t)paths : 0u;
+        uint32_t fixed = level == 0 ? (uint32_t)P.paths : 0u;
         {
-            LaunchTimer t(ctx, RTG_KERNEL_TRACE_CLOSEST, s);
-            if (ctx->tune.refill_closest == 0) {
+            LaunchTimer t(ctx, RTG_KERNEL_TRACE_CLOSEST, s, timed);
+            if (tune.refill_closest == 0) {
                 if (cnt)
                     hipLaunchKernelGGL(k_trace_closest_rounds<true>, dim3(G), dim3(kTraceThreads), 0, s, F,
                                        (int)level, cp, fixed, ctx->d_status);
@@ -1007,7 +1026,7 @@ int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStrea
             }
         }
         {
-            LaunchTimer t(ctx, RTG_KERNEL_SHADE, s);
+            LaunchTimer t(ctx, RTG_KERNEL_SHADE, s, timed);
             if (cnt)
                 hipLaunchKernelGGL(k_shade<true>, dim3(Gsmall), dim3(256), 0, s, F, (int)level, cp, fixed);
             else
@@ -1015,8 +1034,8 @@ int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStrea
         }
     }
     {
-        LaunchTimer t(ctx, RTG_KERNEL_TRACE_OCCLUSION, s);
-        if (ctx->tune.refill_occlusion == 0) {
+        LaunchTimer t(ctx, RTG_KERNEL_TRACE_OCCLUSION, s, timed);
+        if (tune.refill_occlusion == 0) {
             if (cnt)
                 hipLaunchKernelGGL(k_trace_occlusion_rounds<true>, dim3(G), dim3(kTraceThreads), 0, s, F, ctx->d_status);
             else
@@ -1029,10 +1048,21 @@ int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStrea
         }
     }
     {
-        LaunchTimer t(ctx, RTG_KERNEL_RESOLVE, s);
-        unsigned Gr = (unsigned)std::min<uint64_t>((pixels + 255) / 256, (uint64_t)ctx->cu_count * 16);
-        hipLaunchKernelGGL(k_resolve, dim3(Gr), dim3(256), 0, s, F, pixels);
+        LaunchTimer t(ctx, RTG_KERNEL_RESOLVE, s, timed);
+        unsigned Gr = (unsigned)std::min<uint64_t>((P.pixels + 255) / 256, (uint64_t)ctx->cu_count * 16);
+        hipLaunchKernelGGL(k_resolve, dim3(Gr), dim3(256), 0, s, F, P.pixels);
     }
+}
+
+int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStream_t s, FrameArena* a,
+               bool check_tlas) {
+    if (a == nullptr) a = arena(ctx);
+    TracePlan P;
+    int st = plan_jobs(ctx, jobs, njobs, a, check_tlas, ctx->tune, P);
+    if (st != RTG_OK) return st;
+    st = upload_jobs(a, P, s);
+    if (st != RTG_OK) return st;
+    enqueue_trace(ctx, a, P, ctx->tune, ctx->counters_enabled, true, s);
     RTG_HIP(hipGetLastError());
     return RTG_OK;
 }
@@ -1076,7 +1106,7 @@ extern "C" int rtg_trace_rays(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindi
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     RTG_HIP(hipSetDevice(ctx->device));
     FrameArena* a = arena(ctx);
-    if (ensure_spill(ctx, a) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
+    if (ensure_spill(ctx, a, ctx->tune) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
     SceneView S;
     S.models = scene ? reinterpret_cast<const rtg_model_info*>(scene->model_info) : nullptr;
     S.textures = scene ? reinterpret_cast<const rtg_texture*>(scene->textures) : nullptr;

@@ -42,7 +42,14 @@ RTG_QUERY_OCCLUSION = 1
 
 RTG_TRACE_HEATMAP = 0x1
 
-RTG_KERNEL_NAMES = ["raygen", "trace_closest", "shade", "trace_occlusion", "resolve", "refit"]
+RTG_KERNEL_NAMES = ["raygen", "trace_closest", "shade", "trace_occlusion", "resolve", "refit", "batch_graph"]
+RTG_KERNEL_RAYGEN = 0
+RTG_KERNEL_TRACE_CLOSEST = 1
+RTG_KERNEL_SHADE = 2
+RTG_KERNEL_TRACE_OCCLUSION = 3
+RTG_KERNEL_RESOLVE = 4
+RTG_KERNEL_REFIT = 5
+RTG_KERNEL_BATCH_GRAPH = 6
 
 
 class Instance(C.Structure):
@@ -245,6 +252,27 @@ def uniforms_from(view_inverse, proj_inverse, sun_dir, sun_radius, blue_noise_in
     u.frame_index = frame_index
     u.show_heatmap = 1 if show_heatmap else 0
     return u
+
+
+class Tuning(C.Structure):
+    _fields_ = [
+        ("blocks_per_cu", C.c_uint32),
+        ("feed_chunk", C.c_uint32),
+        ("refill_closest", C.c_uint32),
+        ("refill_occlusion", C.c_uint32),
+        ("path_order", C.c_uint32),
+        ("reserved", C.c_uint32 * 3),
+    ]
+
+    def as_dict(self):
+        return {name: int(getattr(self, name)) for name, _ in self._fields_ if name != "reserved"}
+
+    @staticmethod
+    def make(blocks_per_cu=5, feed_chunk=64, refill_closest=64, refill_occlusion=64, path_order=1):
+        return Tuning(blocks_per_cu, feed_chunk, refill_closest, refill_occlusion, path_order)
+
+
+assert C.sizeof(Tuning) == 32
 
 
 class ServerConfig(C.Structure):

@@ -20,6 +20,8 @@ EXPORTS = [
     "rtg_set_counters", "rtg_get_counters", "rtg_set_timing", "rtg_get_kernel_time",
     "rtg_server_create", "rtg_server_add_scene", "rtg_server_submit", "rtg_server_poll", "rtg_server_replay",
     "rtg_server_clock", "rtg_server_get_stats", "rtg_server_destroy",
+    "rtg_get_tuning", "rtg_set_tuning", "rtg_autotune",
+    "rtg_batch_create", "rtg_batch_update", "rtg_batch_launch", "rtg_batch_destroy",
 ]
 
 _lib = None
@@ -68,6 +70,13 @@ def load():
     L.rtg_server_clock.argtypes = [vp]
     L.rtg_server_get_stats.argtypes = [vp, C.POINTER(abi.ServerStats)]
     L.rtg_server_destroy.argtypes = [vp]
+    L.rtg_get_tuning.argtypes = [vp, C.POINTER(abi.Tuning)]
+    L.rtg_set_tuning.argtypes = [vp, C.POINTER(abi.Tuning)]
+    L.rtg_autotune.argtypes = [vp, vp, u32, u32, C.POINTER(abi.Tuning), C.POINTER(C.c_double)]
+    L.rtg_batch_create.argtypes = [vp, vp, u32, C.POINTER(vp)]
+    L.rtg_batch_update.argtypes = [vp, vp, u32, vp]
+    L.rtg_batch_launch.argtypes = [vp, vp]
+    L.rtg_batch_destroy.argtypes = [vp]
     for name in EXPORTS:
         if name not in ("rtg_destroy", "rtg_last_error", "rtg_abi_version"):
             getattr(L, name).restype = C.c_int
@@ -118,6 +127,14 @@ class Context:
         n = C.c_uint64()
         check(load().rtg_get_kernel_time(self.ptr, kind, C.byref(ms), C.byref(n)), "rtg_get_kernel_time")
         return ms.value, n.value
+
+    def tuning(self):
+        t = abi.Tuning()
+        check(load().rtg_get_tuning(self.ptr, C.byref(t)), "rtg_get_tuning")
+        return t
+
+    def set_tuning(self, t):
+        check(load().rtg_set_tuning(self.ptr, C.byref(t)), "rtg_set_tuning")
 
     def as_info(self, address):
         inf = abi.AsInfo()

@@ -585,13 +585,15 @@ class AnimatedScene(Scene):
 
 
 def large_scene(seed=0x5EED0001, width=1920, height=1080, n_objects=1024, n_fences=64, n_mirrors=32,
-                n_soup_meshes=16, soup_tris=1200):
+                n_soup_meshes=16, soup_tris=1200, extent=36.0, p_torus=0.45, p_ico=0.10):
     """The 1080p ~2M-triangle animated scene: n_objects instances drawn from
-    tori (48 x 24), level-4 icospheres (5,120 tris) and random triangle soups
-    (lognormal edge length, median 0.05), n_fences alpha-clipped quads with a
-    256x256 RGBA8 checker alpha, n_mirrors mirror quads, a ground quad; 4
-    blue-noise soft-shadow rays per hit (64x64 mask, 2 degree cone); every
-    frame re-randomises object rotations (TLAS refit)."""
+    tori (48 x 24, probability p_torus), level-4 icospheres (5,120 tris,
+    p_ico) and random triangle soups (lognormal edge length, median 0.05),
+    n_fences alpha-clipped quads with a 256x256 RGBA8 checker alpha,
+    n_mirrors mirror quads, a ground quad; objects spread over
+    [-extent, extent]^2 with the camera scaled to match; 4 blue-noise
+    soft-shadow rays per hit (64x64 mask, 2 degree cone); every frame
+    re-randomises object rotations (TLAS refit)."""
     rng = Rng(seed)
     s = AnimatedScene("large")
     s.seed = seed
@@ -609,7 +611,7 @@ def large_scene(seed=0x5EED0001, width=1920, height=1080, n_objects=1024, n_fenc
     s.instances.append(InstanceData(affine((0.0, 0.0, 0.0)), gm, ground_mesh))
 
     kind = rng.uniform(n_objects)
-    pos = rng.uniform(2 * n_objects, -36.0, 36.0).reshape(-1, 2)
+    pos = rng.uniform(2 * n_objects, -extent, extent).reshape(-1, 2)
     height_ = rng.uniform(n_objects, 0.6, 2.2)
     scale = rng.uniform(n_objects, 0.6, 1.4)
     cols = rng.uniform(3 * n_objects).reshape(-1, 3)
@@ -617,9 +619,9 @@ def large_scene(seed=0x5EED0001, width=1920, height=1080, n_objects=1024, n_fenc
     metal = (rng.uniform(n_objects) < 0.2).astype(np.float64)
     soup_pick = rng.integers(n_objects, 0, n_soup_meshes)
     for k in range(n_objects):
-        if kind[k] < 0.45:
+        if kind[k] < p_torus:
             mesh = t_mesh
-        elif kind[k] < 0.55:
+        elif kind[k] < p_torus + p_ico:
             mesh = i_mesh
         else:
             mesh = soup_meshes[int(soup_pick[k])]
@@ -627,13 +629,14 @@ def large_scene(seed=0x5EED0001, width=1920, height=1080, n_objects=1024, n_fenc
         tr = (pos[k, 0], height_[k], pos[k, 1])
         s.animated.append((len(s.instances), tr, float(scale[k])))
         s.instances.append(InstanceData(affine(tr, None, scale[k]), m, mesh))
-    fpos = rng.uniform(2 * n_fences, -30.0, 30.0).reshape(-1, 2)
+    inner = extent * (30.0 / 36.0)
+    fpos = rng.uniform(2 * n_fences, -inner, inner).reshape(-1, 2)
     fang = rng.uniform(n_fences, 0.0, 2 * math.pi)
     fm = s.add_model(fence_mesh, [(checker, s.material((1, 1, 1), 0.6)[1], -1)])
     for k in range(n_fences):
         s.instances.append(InstanceData(affine((fpos[k, 0], 0.0, fpos[k, 1]), rotation_y(fang[k])), fm, fence_mesh,
                                         abi.RTG_HIT_TEXTURED, abi.RTG_INSTANCE_TRIANGLE_FACING_CULL_DISABLE))
-    mpos = rng.uniform(2 * n_mirrors, -30.0, 30.0).reshape(-1, 2)
+    mpos = rng.uniform(2 * n_mirrors, -inner, inner).reshape(-1, 2)
     mang = rng.uniform(n_mirrors, 0.0, 2 * math.pi)
     mm = s.add_model(mirror_mesh, [s.material((0.9, 0.9, 0.9), 0.1)])
     for k in range(n_mirrors):
@@ -641,7 +644,8 @@ def large_scene(seed=0x5EED0001, width=1920, height=1080, n_objects=1024, n_fenc
                                         mirror_mesh, abi.RTG_HIT_MIRROR,
                                         abi.RTG_INSTANCE_TRIANGLE_FACING_CULL_DISABLE))
     s.width, s.height = width, height
-    s.view_inverse = look_at_inverse((0.0, 16.0, -44.0), (0.0, 0.0, -2.0))
+    cam = extent / 36.0
+    s.view_inverse = look_at_inverse((0.0, 16.0 * cam, -44.0 * cam), (0.0, 0.0, -2.0 * cam))
     s.proj_inverse = proj_inverse_reversed_z(math.radians(60.0), width / height)
     s.sun_dir = sun_direction(0.85, 2.4)
     s.sun_radius = math.tan(math.radians(1.0))  # 2 degree cone
@@ -650,6 +654,44 @@ def large_scene(seed=0x5EED0001, width=1920, height=1080, n_objects=1024, n_fenc
     s.spp = 1
     s.set_frame(0)
     return s
+
+
+BATCH_RESOLUTIONS = ((128, 128), (512, 512), (1280, 720), (1920, 1080))
+
+
+def batch_scene(seed, target_tris, width, height, spp):
+    """One scene of the batched workload with about target_tris instanced
+    triangles: the generator of large_scene with as many objects as the
+    target needs (one triangle soup of target_tris - 2 triangles when it is
+    below a few thousand), fences and mirrors in proportion, and the camera
+    scaled to the spread of the objects."""
+    per_object = 0.45 * 2304 + 0.10 * 5120 + 0.45 * 1200
+    if target_tris < 3 * 2304:
+        s = large_scene(seed, width, height, n_objects=1, n_fences=0, n_mirrors=0, n_soup_meshes=1,
+                        soup_tris=max(1, int(target_tris) - 2), extent=2.0, p_torus=0.0, p_ico=0.0)
+    else:
+        n = max(1, int(round((target_tris - 2) / per_object)))
+        s = large_scene(seed, width, height, n_objects=n, n_fences=min(64, n // 16), n_mirrors=min(32, n // 32),
+                        n_soup_meshes=min(16, n), extent=max(4.0, 36.0 * math.sqrt(n / 1024.0)))
+    s.spp = int(spp)
+    return s
+
+
+def batch_workload(n_scenes=64, seed=0x5EED0003, tri_range=(1_000, 500_000), resolutions=BATCH_RESOLUTIONS,
+                   spp_range=(1, 4)):
+    """The batched launch: n_scenes independent scenes with triangle counts
+    log-uniform in tri_range, resolutions drawn from `resolutions` and
+    samples per pixel uniform in spp_range (inclusive)."""
+    rng = Rng(seed)
+    lo, hi = math.log(tri_range[0]), math.log(tri_range[1])
+    tris = [math.exp(x) for x in rng.uniform(n_scenes, lo, hi)]
+    res = rng.integers(n_scenes, 0, len(resolutions))
+    spp = rng.integers(n_scenes, spp_range[0], spp_range[1] + 1)
+    out = []
+    for k in range(n_scenes):
+        w, h = resolutions[int(res[k])]
+        out.append(batch_scene(seed + 1 + k, float(tris[k]), w, h, int(spp[k])))
+    return out
 
 
 def incoherent_rays(n_rays, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), seed=0x5EED0002):

@@ -65,3 +65,28 @@ def test_context_without_gpu_fails_loudly():
         pytest.skip("GPU present")
     with pytest.raises(lib.RtgError):
         lib.Context(0)
+
+
+def test_python_constants_equal_the_header_defines():
+    text = open(HEADER).read()
+    defines = {m.group(1): int(m.group(2), 0) for m in
+               re.finditer(r"#define\s+(RTG_[A-Z0-9_]+)\s+(0x[0-9a-fA-F]+|\d+)u?\b", text)}
+    checked = 0
+    for name, value in defines.items():
+        if hasattr(abi, name):
+            assert getattr(abi, name) == value, name
+            checked += 1
+    assert checked >= 20
+    kinds = sorted((v, k) for k, v in defines.items() if k.startswith("RTG_KERNEL_") and k != "RTG_KERNEL_COUNT")
+    assert len(abi.RTG_KERNEL_NAMES) == defines["RTG_KERNEL_COUNT"] == len(kinds)
+    for v, k in kinds:
+        assert getattr(abi, k) == v
+
+
+def test_tuning_layout():
+    # rtg_tuning: five u32 settings + three reserved words
+    assert C.sizeof(abi.Tuning) == 32
+    assert abi.Tuning.path_order.offset == 16
+    t = abi.Tuning.make()
+    assert t.as_dict() == {"blocks_per_cu": 5, "feed_chunk": 64, "refill_closest": 64, "refill_occlusion": 64,
+                           "path_order": 1}

@@ -168,3 +168,39 @@ def test_trace_image_multisample(ctx, small):
     o.spp = 3
     o.shadow_rays = 2
     _compare_image(ds, hs, frame=5, opts=o)
+
+
+def test_incoherent_full_scene(ctx):
+    """The incoherent workload's full 1,000,000-triangle BLAS: bytes equal to
+    the oracle's, and 262,144 of its rays (origins in the cube, directions on
+    the sphere, tmax = inf) give identical hits, occlusion and traversal
+    counts in both query kinds."""
+    s = scenes.incoherent_scene()
+    ds = DeviceScene(ctx, s)
+    hs = oh.HostScene(s)
+    assert s.instanced_triangles == 1_000_000
+    g = ds.as_bytes(ds.blas[0])
+    o = np.frombuffer(hs.blas[0].bytes(), dtype=np.uint8)
+    assert np.array_equal(g, o)
+    rays = scenes.incoherent_rays(1 << 18, seed=0x5EED0102)
+    assert np.isinf(rays["tmax"]).all()
+    d = rays_to_device(rays)
+    for mode in (abi.RTG_QUERY_CLOSEST, abi.RTG_QUERY_OCCLUSION):
+        ctx.set_counters(True)
+        try:
+            out = ds.trace_rays(d, len(rays), mode)
+            c = ctx.counters()
+        finally:
+            ctx.set_counters(False)
+        ref, oc = hs.trace_rays(rays, mode=mode)
+        if mode == abi.RTG_QUERY_CLOSEST:
+            got = hits_to_numpy(out)
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+            assert (ref["instance"] != abi.RTG_NO_HIT).mean() > 0.5
+            names = ("node_visits", "triangle_tests", "instance_visits")
+        else:
+            assert np.array_equal(out.cpu().numpy().astype(np.uint32), ref)
+            names = ("occ_node_visits", "occ_triangle_tests", "occ_instance_visits")
+        for f in names:
+            assert getattr(c, f) == getattr(oc, f), f
+    ds.destroy()
