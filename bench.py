@@ -27,14 +27,14 @@ def log(msg):
 
 def algorithmic_bytes(c, kind):
     """Bytes a traversal kernel must move per the path's data layout: the ray
-    (32 B) and its result (24 B hit / 4 B flag), 64 B per node record
+    (32 B) and its result (24 B hit / 4 B flag), 128 B per 4-wide node record
     visited, 48 B per triangle record tested, 64 B instance record + 48 B
     world-to-object transform per BLAS entered."""
     if kind == "closest":
         rays = c["primary_rays"] + c["bounce_rays"]
-        return rays * (32 + 24) + c["node_visits"] * 64 + c["triangle_tests"] * 48 + c["instance_visits"] * 112
+        return rays * (32 + 24) + c["node_visits"] * 128 + c["triangle_tests"] * 48 + c["instance_visits"] * 112
     rays = c["shadow_rays"]
-    return rays * (32 + 4) + c["occ_node_visits"] * 64 + c["occ_triangle_tests"] * 48 + c["occ_instance_visits"] * 112
+    return rays * (32 + 4) + c["occ_node_visits"] * 128 + c["occ_triangle_tests"] * 48 + c["occ_instance_visits"] * 112
 
 
 def cpu_baseline(scene, threads):
@@ -263,10 +263,10 @@ def incoherent_benchmark(ctx, n_rays=16 << 20, sample=65536):
         c = ctx.counters()
         ctx.set_counters(False)
         if name == "closest":
-            nb = n_rays * (32 + 24) + c.node_visits * 64 + c.triangle_tests * 48 + c.instance_visits * 112
+            nb = n_rays * (32 + 24) + c.node_visits * 128 + c.triangle_tests * 48 + c.instance_visits * 112
             nodes, trisv = c.node_visits, c.triangle_tests
         else:
-            nb = n_rays * (32 + 4) + c.occ_node_visits * 64 + c.occ_triangle_tests * 48 + c.occ_instance_visits * 112
+            nb = n_rays * (32 + 4) + c.occ_node_visits * 128 + c.occ_triangle_tests * 48 + c.occ_instance_visits * 112
             nodes, trisv = c.occ_node_visits, c.occ_triangle_tests
         out[name] = {"ms": round(ms, 3), "mrays_s": round(n_rays / (ms / 1e3) / 1e6, 1),
                      "nodes_per_ray": round(nodes / n_rays, 2), "tris_per_ray": round(trisv / n_rays, 2),
@@ -371,6 +371,7 @@ def main():
         c = ctx.counters().as_dict()
         tot = c if tot is None else {k: tot[k] + c[k] for k in tot}
     ctx.set_counters(False)
+    overflow = bool(ctx.status() & abi.RTG_STATUS_STACK_OVERFLOW)
     rays = tot["primary_rays"] + tot["bounce_rays"] + tot["shadow_rays"]
     elapsed = t1 - t0
     value = rays / elapsed / 1e6
@@ -401,7 +402,8 @@ def main():
               "roofline": roofline, "kernel_ms_per_frame": kernel_ms,
               "traversal": {"nodes_per_closest_ray": round(tot["node_visits"] / max(1, tot["primary_rays"] +
                                                                                     tot["bounce_rays"]), 2),
-                            "nodes_per_shadow_ray": round(tot["occ_node_visits"] / max(1, tot["shadow_rays"]), 2)}}
+                            "nodes_per_shadow_ray": round(tot["occ_node_visits"] / max(1, tot["shadow_rays"]), 2),
+                            "node_width": 4, "stack_overflow": overflow}}
 
     if not args.no_serve:
         t_s = time.perf_counter()

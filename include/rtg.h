@@ -344,6 +344,15 @@ typedef struct rtg_trace_counters {
 int rtg_set_counters(rtg_ctx* ctx, int enable);
 int rtg_get_counters(rtg_ctx* ctx, rtg_trace_counters* out);
 
+/* Sticky device status bits of the context, read after all work queued so
+ * far has finished (synchronises); clear != 0 resets them. A trace that sets
+ * RTG_STATUS_STACK_OVERFLOW has rays whose traversal stack (96 entries) ran
+ * out: their results are not valid. */
+#define RTG_STATUS_BAD_INDEX 0x1u      /* BLAS build: an index above max_vertex */
+#define RTG_STATUS_UNKNOWN_BLAS 0x2u   /* TLAS build/refit: a reference that is not a live BLAS */
+#define RTG_STATUS_STACK_OVERFLOW 0x4u /* trace: traversal stack exhausted */
+int rtg_get_status(rtg_ctx* ctx, uint32_t* bits, int clear);
+
 /* Per-kernel-class GPU time (HIP events on the launch stream), accumulated
  * while timing is enabled. kind: RTG_KERNEL_*; returns total ms and launches. */
 #define RTG_KERNEL_RAYGEN 0u

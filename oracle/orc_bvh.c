@@ -104,8 +104,9 @@ static int delta(const uint32_t* keys, int n, int i, int j) {
 }
 
 /* Radix-tree node i over n sorted keys: children as refs (leaf bit set for
- * leaves). Karras 2012, Figure 4. */
-static void karras_node(const uint32_t* keys, int n, int i, uint32_t* left, uint32_t* right) {
+ * leaves) and the leaf range the node covers. Karras 2012, Figure 4. */
+static void karras_node(const uint32_t* keys, int n, int i, uint32_t* left, uint32_t* right, uint32_t* first,
+                        uint32_t* count) {
     int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
     int dmin = delta(keys, n, i, i - d);
     int lmax = 2;
@@ -126,56 +127,141 @@ static void karras_node(const uint32_t* keys, int n, int i, uint32_t* left, uint
     int hi = i < j ? j : i;
     *left = (lo == gamma) ? (ORC_LEAF_BIT | (uint32_t)gamma) : (uint32_t)gamma;
     *right = (hi == gamma + 1) ? (ORC_LEAF_BIT | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+    *first = (uint32_t)lo;
+    *count = (uint32_t)(hi - lo + 1);
 }
 
-/* Output of the generic LBVH over n primitive boxes. */
+/* Surface area of a box (half of it: the heuristic only compares). */
+static float box_area(box3 b) {
+    float ex = b.hi[0] - b.lo[0], ey = b.hi[1] - b.lo[1], ez = b.hi[2] - b.lo[2];
+    return (ex * ey + ey * ez) + ez * ex;
+}
+
+/* The binary radix tree over n >= 2 sorted primitives. */
+typedef struct {
+    uint32_t* lc;     /* n - 1 internal nodes: child refs */
+    uint32_t* rc;
+    uint32_t* first;  /* leaf range */
+    uint32_t* count;
+    float* area;      /* box surface area */
+} bintree;
+
+/* The 4-wide tree of a build. */
 typedef struct {
     uint32_t n;
+    uint32_t* order;      /* order[k] = primitive of leaf k */
+    orc_node* nodes;      /* breadth-first */
     uint32_t node_count;
-    uint32_t* order;    /* order[k] = primitive of leaf k */
-    orc_node* nodes;    /* node_count */
+    uint32_t* level_off;  /* level_count + 1 */
+    uint32_t level_count;
     box3 root;
-    uint32_t max_depth;
-    uint32_t* depth;    /* internal node depth (root = 0) */
-} lbvh;
+} wtree;
 
-static void set_child(orc_node* nd, int slot, uint32_t ref, box3 b) {
-    nd->x[2 * slot] = b.lo[0];
-    nd->x[2 * slot + 1] = b.hi[0];
-    nd->y[2 * slot] = b.lo[1];
-    nd->y[2 * slot + 1] = b.hi[1];
-    nd->z[2 * slot] = b.lo[2];
-    nd->z[2 * slot + 1] = b.hi[2];
-    nd->child[slot] = ref;
+static void wtree_free(wtree* w) {
+    free(w->order);
+    free(w->nodes);
+    free(w->level_off);
 }
 
-static box3 node_box(const orc_node* nd) {
+static void set_slot(orc_node* nd, int s, uint32_t ref, box3 b) {
+    nd->lo_x[s] = b.lo[0];
+    nd->hi_x[s] = b.hi[0];
+    nd->lo_y[s] = b.lo[1];
+    nd->hi_y[s] = b.hi[1];
+    nd->lo_z[s] = b.lo[2];
+    nd->hi_z[s] = b.hi[2];
+    nd->child[s] = ref;
+}
+
+static box3 slot_box(const orc_node* nd, int s) {
     box3 b;
-    b.lo[0] = mins(nd->x[0], nd->x[2]);
-    b.hi[0] = maxs(nd->x[1], nd->x[3]);
-    b.lo[1] = mins(nd->y[0], nd->y[2]);
-    b.hi[1] = maxs(nd->y[1], nd->y[3]);
-    b.lo[2] = mins(nd->z[0], nd->z[2]);
-    b.hi[2] = maxs(nd->z[1], nd->z[3]);
+    b.lo[0] = nd->lo_x[s];
+    b.hi[0] = nd->hi_x[s];
+    b.lo[1] = nd->lo_y[s];
+    b.hi[1] = nd->hi_y[s];
+    b.lo[2] = nd->lo_z[s];
+    b.hi[2] = nd->hi_z[s];
     return b;
 }
 
-static int lbvh_build(const box3* prim_boxes, uint32_t n, lbvh* out) {
+/* Box of a whole node: its four slot boxes folded in slot order. */
+static box3 node_box(const orc_node* nd) {
+    box3 b = box_empty();
+    for (int s = 0; s < 4; ++s) b = box_union(b, slot_box(nd, s));
+    return b;
+}
+
+/* Slot box of child ref c: its leaf's primitive boxes folded in leaf order,
+ * or its node's box. */
+static box3 ref_box(const orc_node* nodes, uint32_t c, const box3* leaf_boxes) {
+    box3 b = box_empty();
+    if (c == ORC_EMPTY) return b;
+    if (c & ORC_LEAF_BIT) {
+        uint32_t f = c & 0x07ffffffu, cnt = ((c >> 27) & 0xfu) + 1u;
+        for (uint32_t k = f; k < f + cnt; ++k) b = box_union(b, leaf_boxes[k]);
+        return b;
+    }
+    return node_box(&nodes[c]);
+}
+
+/* Slots of the wide node rooted at binary node r: r's children, then the
+ * largest-area openable slot replaced by its two children while fewer than
+ * four. Returns the slot count. */
+static int expand_slots(const bintree* t, uint32_t r, uint32_t leaf_max, uint32_t s[4]) {
+    int m = 2;
+    s[0] = t->lc[r];
+    s[1] = t->rc[r];
+    while (m < 4) {
+        int best = -1;
+        float best_area = 0.0f;
+        for (int k = 0; k < m; ++k) {
+            uint32_t c = s[k];
+            if ((c & ORC_LEAF_BIT) || t->count[c] <= leaf_max) continue;
+            if (best < 0 || t->area[c] > best_area) {
+                best = k;
+                best_area = t->area[c];
+            }
+        }
+        if (best < 0) break;
+        uint32_t c = s[best];
+        for (int k = m; k > best + 1; --k) s[k] = s[k - 1];
+        s[best] = t->lc[c];
+        s[best + 1] = t->rc[c];
+        ++m;
+    }
+    return m;
+}
+
+/* Builds the wide tree over n primitive boxes. */
+static int wide_build(const box3* prim_boxes, uint32_t n, uint32_t leaf_max, wtree* out) {
     memset(out, 0, sizeof(*out));
     out->n = n;
-    out->node_count = n >= 2 ? n - 1 : 1;
+    uint32_t cap = n >= 2 ? n - 1 : 1;
     out->order = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
-    out->nodes = (orc_node*)calloc(out->node_count, sizeof(orc_node));
-    out->depth = (uint32_t*)calloc(out->node_count, sizeof(uint32_t));
-    if (!out->order || !out->nodes || !out->depth) return RTG_ERROR_OUT_OF_MEMORY;
+    out->nodes = (orc_node*)calloc(cap, sizeof(orc_node));
+    out->level_off = (uint32_t*)calloc(cap + 2, sizeof(uint32_t));
+    box3* leaf_boxes = (box3*)malloc(sizeof(box3) * (n ? n : 1));
+    if (!out->order || !out->nodes || !out->level_off || !leaf_boxes) {
+        free(leaf_boxes);
+        return RTG_ERROR_OUT_OF_MEMORY;
+    }
+    for (uint32_t i = 0; i < n; ++i) out->order[i] = i;
 
-    if (n <= 1) {
+    if (n <= leaf_max || n <= 1) {
+        /* one node with one leaf (or none) */
         box3 e = box_empty();
-        set_child(&out->nodes[0], 0, n == 1 ? ORC_LEAF_BIT : ORC_EMPTY, n == 1 ? prim_boxes[0] : e);
-        set_child(&out->nodes[0], 1, ORC_EMPTY, e);
-        if (n == 1) out->order[0] = 0;
-        out->root = n == 1 ? prim_boxes[0] : e;
-        out->max_depth = 1;
+        for (int s = 0; s < 4; ++s) set_slot(&out->nodes[0], s, ORC_EMPTY, e);
+        if (n > 0) {
+            uint32_t ref = ORC_LEAF_BIT | ((n - 1) << 27);
+            for (uint32_t k = 0; k < n; ++k) leaf_boxes[k] = prim_boxes[k];
+            set_slot(&out->nodes[0], 0, ref, ref_box(out->nodes, ref, leaf_boxes));
+        }
+        out->node_count = 1;
+        out->level_count = 1;
+        out->level_off[0] = 0;
+        out->level_off[1] = 1;
+        out->root = node_box(&out->nodes[0]);
+        free(leaf_boxes);
         return RTG_OK;
     }
 
@@ -193,21 +279,24 @@ static int lbvh_build(const box3* prim_boxes, uint32_t n, lbvh* out) {
             }
         }
     uint32_t* keys = (uint32_t*)malloc(sizeof(uint32_t) * n);
-    for (uint32_t i = 0; i < n; ++i) {
-        keys[i] = orc_morton3(cen[3 * i], cen[3 * i + 1], cen[3 * i + 2], clo, chi);
-        out->order[i] = i;
-    }
+    for (uint32_t i = 0; i < n; ++i) keys[i] = orc_morton3(cen[3 * i], cen[3 * i + 1], cen[3 * i + 2], clo, chi);
     free(cen);
     radix_sort_pairs(keys, out->order, n);
+    for (uint32_t k = 0; k < n; ++k) leaf_boxes[k] = prim_boxes[out->order[k]];
 
+    /* binary radix tree and the boxes of its internal nodes */
     uint32_t nint = n - 1;
-    uint32_t* lc = (uint32_t*)malloc(sizeof(uint32_t) * nint);
-    uint32_t* rc = (uint32_t*)malloc(sizeof(uint32_t) * nint);
+    bintree t;
+    t.lc = (uint32_t*)malloc(sizeof(uint32_t) * nint);
+    t.rc = (uint32_t*)malloc(sizeof(uint32_t) * nint);
+    t.first = (uint32_t*)malloc(sizeof(uint32_t) * nint);
+    t.count = (uint32_t*)malloc(sizeof(uint32_t) * nint);
+    t.area = (float*)malloc(sizeof(float) * nint);
     uint32_t* parent_int = (uint32_t*)malloc(sizeof(uint32_t) * nint);
     uint32_t* parent_leaf = (uint32_t*)malloc(sizeof(uint32_t) * n);
     for (uint32_t i = 0; i < nint; ++i) {
-        karras_node(keys, (int)n, (int)i, &lc[i], &rc[i]);
-        uint32_t c[2] = {lc[i], rc[i]};
+        karras_node(keys, (int)n, (int)i, &t.lc[i], &t.rc[i], &t.first[i], &t.count[i]);
+        uint32_t c[2] = {t.lc[i], t.rc[i]};
         for (int s = 0; s < 2; ++s) {
             if (c[s] & ORC_LEAF_BIT) parent_leaf[c[s] & ~ORC_LEAF_BIT] = i;
             else parent_int[c[s]] = i;
@@ -215,76 +304,90 @@ static int lbvh_build(const box3* prim_boxes, uint32_t n, lbvh* out) {
     }
     parent_int[0] = 0xffffffffu;
     free(keys);
-
-    /* bottom-up boxes: a node is finished when both children are */
+    /* bottom-up: a node is finished when both children are */
     box3* ibox = (box3*)malloc(sizeof(box3) * nint);
-    uint32_t* height = (uint32_t*)calloc(nint, sizeof(uint32_t));
     uint8_t* arrivals = (uint8_t*)calloc(nint, 1);
     for (uint32_t k = 0; k < n; ++k) {
         uint32_t node = parent_leaf[k];
         while (node != 0xffffffffu) {
-            if (arrivals[node]++ == 0) break; /* first child to arrive stops */
+            if (arrivals[node]++ == 0) break; /* the first child to arrive stops */
+            uint32_t c[2] = {t.lc[node], t.rc[node]};
             box3 cb[2];
-            uint32_t ch[2];
-            uint32_t hmax = 0;
-            uint32_t c[2] = {lc[node], rc[node]};
-            for (int s = 0; s < 2; ++s) {
-                if (c[s] & ORC_LEAF_BIT) {
-                    cb[s] = prim_boxes[out->order[c[s] & ~ORC_LEAF_BIT]];
-                    ch[s] = 0;
-                } else {
-                    cb[s] = ibox[c[s]];
-                    ch[s] = height[c[s]];
-                }
-                if (ch[s] > hmax) hmax = ch[s];
-            }
+            for (int s = 0; s < 2; ++s)
+                cb[s] = (c[s] & ORC_LEAF_BIT) ? leaf_boxes[c[s] & ~ORC_LEAF_BIT] : ibox[c[s]];
             ibox[node] = box_union(cb[0], cb[1]);
-            height[node] = hmax + 1;
-            set_child(&out->nodes[node], 0, c[0], cb[0]);
-            set_child(&out->nodes[node], 1, c[1], cb[1]);
+            t.area[node] = box_area(ibox[node]);
             node = parent_int[node];
         }
     }
-    out->root = ibox[0];
-    out->max_depth = height[0];
-    /* depth of each internal node (root = 0); parents have smaller indices? no:
-       walk up the parent chain */
-    for (uint32_t i = 0; i < nint; ++i) {
-        uint32_t dpt = 0, p = parent_int[i];
-        while (p != 0xffffffffu) {
-            ++dpt;
-            p = parent_int[p];
-        }
-        out->depth[i] = dpt;
-    }
-    free(lc);
-    free(rc);
+    free(ibox);
+    free(arrivals);
     free(parent_int);
     free(parent_leaf);
-    free(ibox);
-    free(height);
-    free(arrivals);
+
+    /* top-down collapse, breadth first */
+    uint32_t* root_bin = (uint32_t*)malloc(sizeof(uint32_t) * cap);
+    root_bin[0] = 0;
+    uint32_t lb = 0, le = 1, levels = 0;
+    while (lb < le) {
+        out->level_off[levels++] = lb;
+        uint32_t next = le;
+        for (uint32_t w = lb; w < le; ++w) {
+            uint32_t s[4];
+            int m = expand_slots(&t, root_bin[w], leaf_max, s);
+            for (int k = 0; k < 4; ++k) {
+                uint32_t ref = ORC_EMPTY;
+                if (k < m) {
+                    uint32_t c = s[k];
+                    if (c & ORC_LEAF_BIT) {
+                        ref = c;
+                    } else if (t.count[c] <= leaf_max) {
+                        ref = ORC_LEAF_BIT | ((t.count[c] - 1) << 27) | t.first[c];
+                    } else {
+                        ref = next;
+                        root_bin[next++] = c;
+                    }
+                }
+                out->nodes[w].child[k] = ref;
+            }
+        }
+        lb = le;
+        le = next;
+    }
+    out->level_off[levels] = le;
+    out->level_count = levels;
+    out->node_count = le;
+    free(root_bin);
+    free(t.lc);
+    free(t.rc);
+    free(t.first);
+    free(t.count);
+    free(t.area);
+
+    /* slot boxes, deepest level first */
+    for (uint32_t L = levels; L-- > 0;)
+        for (uint32_t w = out->level_off[L]; w < out->level_off[L + 1]; ++w)
+            for (int k = 0; k < 4; ++k) {
+                uint32_t c = out->nodes[w].child[k];
+                set_slot(&out->nodes[w], k, c, ref_box(out->nodes, c, leaf_boxes));
+            }
+    out->root = node_box(&out->nodes[0]);
+    free(leaf_boxes);
     return RTG_OK;
 }
 
-static void lbvh_free(lbvh* b) {
-    free(b->order);
-    free(b->nodes);
-    free(b->depth);
-}
-
-static void write_header(orc_header* h, uint32_t kind, const lbvh* b, uint32_t geom_count) {
+static void write_header(orc_header* h, uint32_t kind, const wtree* w, uint32_t geom_count) {
     memset(h, 0, sizeof(*h));
     h->magic = ORC_MAGIC;
     h->kind = kind;
-    h->node_count = b->node_count;
-    h->prim_count = b->n;
+    h->node_count = w->node_count;
+    h->prim_count = w->n;
     h->geom_count = geom_count;
-    h->max_depth = b->max_depth;
-    h->node_width = 2;
+    h->max_depth = w->level_count;
+    h->node_width = 4;
     for (int a = 0; a < 3; ++a) {
-        h->lo[a] = b->root.lo[a];
-        h->hi[a] = b->root.hi[a];
+        h->lo[a] = w->root.lo[a];
+        h->hi[a] = w->root.hi[a];
     }
 }
 
@@ -330,31 +433,31 @@ int orc_blas_build(const rtg_geometry* geoms, uint32_t geom_count, void** out_bl
             boxes[p] = bx;
         }
     }
-    lbvh b;
-    int st = lbvh_build(boxes, n, &b);
+    wtree w;
+    int st = wide_build(boxes, n, ORC_BLAS_LEAF_MAX, &w);
     if (st) {
         free(boxes);
         free(tris);
-        lbvh_free(&b);
+        wtree_free(&w);
         return st;
     }
     uint64_t nodes_off = 128;
-    uint64_t prims_off = orc_align(nodes_off + (uint64_t)b.node_count * 64, 64);
+    uint64_t prims_off = orc_align(nodes_off + (uint64_t)w.node_count * sizeof(orc_node), 64);
     uint64_t extra_off = orc_align(prims_off + (uint64_t)n * 48, 64);
     uint64_t size = orc_align(extra_off + (uint64_t)geom_count * 4, 64);
     uint8_t* blob = (uint8_t*)calloc(1, size);
     orc_header* h = (orc_header*)blob;
-    write_header(h, ORC_KIND_BLAS, &b, geom_count);
+    write_header(h, ORC_KIND_BLAS, &w, geom_count);
     h->nodes_off = nodes_off;
     h->prims_off = prims_off;
     h->extra_off = extra_off;
     h->total_size = size;
-    memcpy(blob + nodes_off, b.nodes, (size_t)b.node_count * 64);
+    memcpy(blob + nodes_off, w.nodes, (size_t)w.node_count * sizeof(orc_node));
     orc_tri* dst = (orc_tri*)(blob + prims_off);
-    for (uint32_t k = 0; k < n; ++k) dst[k] = tris[b.order[k]];
+    for (uint32_t k = 0; k < n; ++k) dst[k] = tris[w.order[k]];
     uint32_t* gflags = (uint32_t*)(blob + extra_off);
     for (uint32_t g = 0; g < geom_count; ++g) gflags[g] = geoms[g].flags;
-    lbvh_free(&b);
+    wtree_free(&w);
     free(boxes);
     free(tris);
     *out_blob = blob;
@@ -390,54 +493,40 @@ int orc_tlas_build(const rtg_instance* inst, uint32_t count, void** out_blob, ui
     if (count >= (1u << 27)) return RTG_ERROR_LIMIT;
     box3* boxes = (box3*)calloc(count ? count : 1, sizeof(box3));
     for (uint32_t i = 0; i < count; ++i) boxes[i] = instance_box(&inst[i]);
-    lbvh b;
-    int st = lbvh_build(boxes, count, &b);
+    wtree w;
+    int st = wide_build(boxes, count, ORC_TLAS_LEAF_MAX, &w);
     free(boxes);
     if (st) {
-        lbvh_free(&b);
+        wtree_free(&w);
         return st;
     }
-    uint32_t nint = count >= 2 ? count - 1 : 0;
-    uint32_t levels = 0;
-    for (uint32_t i = 0; i < nint; ++i)
-        if (b.depth[i] + 1 > levels) levels = b.depth[i] + 1;
     uint64_t nodes_off = 128;
-    uint64_t prims_off = orc_align(nodes_off + (uint64_t)b.node_count * 64, 64);
+    uint64_t prims_off = orc_align(nodes_off + (uint64_t)w.node_count * sizeof(orc_node), 64);
     uint64_t extra_off = orc_align(prims_off + (uint64_t)count * 64, 64);
     uint64_t level_off = orc_align(extra_off + (uint64_t)count * 64, 64);
-    uint64_t leafbox_off = orc_align(level_off + 4ull * (1 + (nint + 1) + nint), 64);
+    uint64_t leafbox_off = orc_align(level_off + 4ull * (2 + w.level_count), 64);
     uint64_t size = orc_align(leafbox_off + (uint64_t)count * 24, 64);
     uint8_t* blob = (uint8_t*)calloc(1, size);
     orc_header* h = (orc_header*)blob;
-    write_header(h, ORC_KIND_TLAS, &b, 0);
+    write_header(h, ORC_KIND_TLAS, &w, 0);
     h->nodes_off = nodes_off;
     h->prims_off = prims_off;
     h->extra_off = extra_off;
     h->level_off = level_off;
     h->leafbox_off = leafbox_off;
     h->total_size = size;
-    memcpy(blob + nodes_off, b.nodes, (size_t)b.node_count * 64);
+    memcpy(blob + nodes_off, w.nodes, (size_t)w.node_count * sizeof(orc_node));
     orc_inst* recs = (orc_inst*)(blob + prims_off);
     box3* leafbox = (box3*)(blob + leafbox_off);
     for (uint32_t k = 0; k < count; ++k) {
-        fill_inst_record(&recs[k], &inst[b.order[k]], b.order[k]);
-        leafbox[k] = instance_box(&inst[b.order[k]]);
+        fill_inst_record(&recs[k], &inst[w.order[k]], w.order[k]);
+        leafbox[k] = instance_box(&inst[w.order[k]]);
     }
     memcpy(blob + extra_off, inst, (size_t)count * 64);
-    /* level lists: deepest level first, ascending node index within a level */
     uint32_t* lv = (uint32_t*)(blob + level_off);
-    lv[0] = levels;
-    uint32_t* offs = lv + 1;
-    uint32_t* list = lv + 2 + levels;
-    uint32_t pos = 0;
-    for (uint32_t L = 0; L < levels; ++L) {
-        uint32_t depth = levels - 1 - L;
-        offs[L] = pos;
-        for (uint32_t i = 0; i < nint; ++i)
-            if (b.depth[i] == depth) list[pos++] = i;
-    }
-    offs[levels] = pos;
-    lbvh_free(&b);
+    lv[0] = w.level_count;
+    for (uint32_t L = 0; L <= w.level_count; ++L) lv[1 + L] = w.level_off[L];
+    wtree_free(&w);
     *out_blob = blob;
     *out_size = size;
     return RTG_OK;
@@ -456,27 +545,17 @@ int orc_tlas_refit(void* blob_v, const rtg_instance* inst, uint32_t count) {
         fill_inst_record(&recs[k], &inst[src], src);
         leaf[k] = instance_box(&inst[src]);
     }
-    box3 root;
-    if (count <= 1) {
-        box3 e = box_empty();
-        if (count == 1) set_child(&nodes[0], 0, ORC_LEAF_BIT, leaf[0]);
-        root = count == 1 ? leaf[0] : e;
-    } else {
-        const uint32_t* lv = (const uint32_t*)(blob + h->level_off);
-        uint32_t levels = lv[0];
-        const uint32_t* offs = lv + 1;
-        const uint32_t* list = lv + 2 + levels;
-        for (uint32_t L = 0; L < levels; ++L)
-            for (uint32_t q = offs[L]; q < offs[L + 1]; ++q) {
-                orc_node* nd = &nodes[list[q]];
-                for (int s = 0; s < 2; ++s) {
-                    uint32_t c = nd->child[s];
-                    box3 cb = (c & ORC_LEAF_BIT) ? leaf[c & 0x07ffffffu] : node_box(&nodes[c]);
-                    set_child(nd, s, c, cb);
-                }
+    /* same topology: slot boxes again, deepest level first */
+    const uint32_t* lv = (const uint32_t*)(blob + h->level_off);
+    uint32_t levels = lv[0];
+    const uint32_t* offs = lv + 1;
+    for (uint32_t L = levels; L-- > 0;)
+        for (uint32_t q = offs[L]; q < offs[L + 1]; ++q)
+            for (int s = 0; s < 4; ++s) {
+                uint32_t c = nodes[q].child[s];
+                set_slot(&nodes[q], s, c, ref_box(nodes, c, leaf));
             }
-        root = node_box(&nodes[0]);
-    }
+    box3 root = node_box(&nodes[0]);
     for (int a = 0; a < 3; ++a) {
         h->lo[a] = root.lo[a];
         h->hi[a] = root.hi[a];

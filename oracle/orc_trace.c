@@ -195,12 +195,12 @@ void orc_watertight(const float o[3], const float d[3], const float v0[3], const
 static int box_test(const orc_node* nd, int s, const oray* r, float tmin, float tcur, float* tn) {
     /* near/far planes by the sign bit of the direction (-0 selects like < 0,
        matching 1/-0 = -inf) */
-    float nx = signbit(r->d.x) ? nd->x[2 * s + 1] : nd->x[2 * s];
-    float fx = signbit(r->d.x) ? nd->x[2 * s] : nd->x[2 * s + 1];
-    float ny = signbit(r->d.y) ? nd->y[2 * s + 1] : nd->y[2 * s];
-    float fy = signbit(r->d.y) ? nd->y[2 * s] : nd->y[2 * s + 1];
-    float nz = signbit(r->d.z) ? nd->z[2 * s + 1] : nd->z[2 * s];
-    float fz = signbit(r->d.z) ? nd->z[2 * s] : nd->z[2 * s + 1];
+    float nx = signbit(r->d.x) ? nd->hi_x[s] : nd->lo_x[s];
+    float fx = signbit(r->d.x) ? nd->lo_x[s] : nd->hi_x[s];
+    float ny = signbit(r->d.y) ? nd->hi_y[s] : nd->lo_y[s];
+    float fy = signbit(r->d.y) ? nd->lo_y[s] : nd->hi_y[s];
+    float nz = signbit(r->d.z) ? nd->hi_z[s] : nd->lo_z[s];
+    float fz = signbit(r->d.z) ? nd->lo_z[s] : nd->hi_z[s];
     float tnx = (nx - r->o.x) * r->inv.x, tfx = (fx - r->o.x) * r->inv.x;
     float tny = (ny - r->o.y) * r->inv.y, tfy = (fy - r->o.y) * r->inv.y;
     float tnz = (nz - r->o.z) * r->inv.z, tfz = (fz - r->o.z) * r->inv.z;
@@ -273,23 +273,35 @@ static int traverse(qctx* q, v3 o, v3 d, float tmin, float tmax, int occlusion, 
             if (!(cur & ORC_LEAF_BIT)) {
                 const orc_node* nd = &nodes[cur];
                 kc[0]++;
-                float tn0 = 0.0f, tn1 = 0.0f;
-                int h0 = nd->child[0] != ORC_EMPTY && box_test(nd, 0, cr, tmin, tcur, &tn0);
-                int h1 = nd->child[1] != ORC_EMPTY && box_test(nd, 1, cr, tmin, tcur, &tn1);
-                if (h0 && h1) {
-                    int first = tn1 < tn0 ? 1 : 0;
-                    cur = nd->child[first];
-                    if (sp >= STACK_MAX) return -1;
-                    stack_ref[sp] = nd->child[1 - first];
-                    stack_tn[sp] = first ? tn0 : tn1;
-                    ++sp;
-                } else if (h0) {
-                    cur = nd->child[0];
-                } else if (h1) {
-                    cur = nd->child[1];
-                } else {
-                    have = 0;
+                /* the overlapped slots in ascending (entry distance, slot)
+                   order: the first is visited next, the others are pushed
+                   so that the second is on top */
+                uint32_t ref[4];
+                float tn[4];
+                int nh = 0;
+                for (int s = 0; s < 4; ++s) {
+                    float t = 0.0f;
+                    if (nd->child[s] == ORC_EMPTY || !box_test(nd, s, cr, tmin, tcur, &t)) continue;
+                    int k = nh++;
+                    while (k > 0 && t < tn[k - 1]) {
+                        ref[k] = ref[k - 1];
+                        tn[k] = tn[k - 1];
+                        --k;
+                    }
+                    ref[k] = nd->child[s];
+                    tn[k] = t;
                 }
+                if (nh == 0) {
+                    have = 0;
+                    continue;
+                }
+                for (int k = nh - 1; k >= 1; --k) {
+                    if (sp >= STACK_MAX) return -1;
+                    stack_ref[sp] = ref[k];
+                    stack_tn[sp] = tn[k];
+                    ++sp;
+                }
+                cur = ref[0];
                 continue;
             }
             if (!in_blas) {

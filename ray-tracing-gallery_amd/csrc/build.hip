@@ -7,9 +7,13 @@
 //   AccelerationStructure::clone        src/util_structs.rs:359-405
 // with a linear BVH (Karras 2012): primitive boxes and centres, 30-bit Morton
 // codes, a stable LSD radix sort written here (LDS digit ranking with wave
-// ballots), the radix-tree split search, and a bottom-up box merge with
-// atomic arrival counters. The node arrays are bit-identical to the host
-// oracle's: every box operation is an exact min/max or a fixed FP32 sequence.
+// ballots), the radix-tree split search and a bottom-up box merge with
+// atomic arrival counters, then a top-down collapse of the binary tree into
+// 4-wide nodes, one breadth-first level per launch (node numbers from a
+// prefix sum, so they do not depend on scheduling), and the slot boxes of
+// the wide nodes from the deepest level up. The blobs are bit-identical to
+// the host oracle's: every box operation is an exact min/max or a fixed FP32
+// sequence.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -51,25 +55,51 @@ __device__ inline float ord2f(uint32_t o) { return __uint_as_float((o & 0x800000
 
 __device__ inline float centre(const Box& b, int a) { return (b.lo[a] + b.hi[a]) * 0.5f + 0.0f; }
 
-__device__ inline void set_child(Node& nd, int s, uint32_t ref, const Box& b) {
-    nd.x[2 * s] = b.lo[0];
-    nd.x[2 * s + 1] = b.hi[0];
-    nd.y[2 * s] = b.lo[1];
-    nd.y[2 * s + 1] = b.hi[1];
-    nd.z[2 * s] = b.lo[2];
-    nd.z[2 * s + 1] = b.hi[2];
-    nd.child[s] = ref;
+__device__ inline void set_slot(Node& nd, int s, const Box& b) {
+    nd.lo_x[s] = b.lo[0];
+    nd.hi_x[s] = b.hi[0];
+    nd.lo_y[s] = b.lo[1];
+    nd.hi_y[s] = b.hi[1];
+    nd.lo_z[s] = b.lo[2];
+    nd.hi_z[s] = b.hi[2];
 }
 
-__device__ inline Box node_box(const Node& nd) {
+__device__ inline Box slot_box(const Node& nd, int s) {
     Box b;
-    b.lo[0] = mins(nd.x[0], nd.x[2]);
-    b.hi[0] = maxs(nd.x[1], nd.x[3]);
-    b.lo[1] = mins(nd.y[0], nd.y[2]);
-    b.hi[1] = maxs(nd.y[1], nd.y[3]);
-    b.lo[2] = mins(nd.z[0], nd.z[2]);
-    b.hi[2] = maxs(nd.z[1], nd.z[3]);
+    b.lo[0] = nd.lo_x[s];
+    b.hi[0] = nd.hi_x[s];
+    b.lo[1] = nd.lo_y[s];
+    b.hi[1] = nd.hi_y[s];
+    b.lo[2] = nd.lo_z[s];
+    b.hi[2] = nd.hi_z[s];
     return b;
+}
+
+// Box of a node: its four slot boxes folded in slot order.
+__device__ inline Box node_box(const Node& nd) {
+    Box b = box_empty();
+#pragma unroll
+    for (int s = 0; s < 4; ++s) b = box_union(b, slot_box(nd, s));
+    return b;
+}
+
+// Slot box of child ref c: its leaf's primitive boxes (leaf_boxes[order[k]],
+// or leaf_boxes[k] without an order) folded in leaf order, or its node's box.
+__device__ inline Box ref_box(const Node* nodes, uint32_t c, const Box* leaf_boxes, const uint32_t* order) {
+    Box b = box_empty();
+    if (c == kEmpty) return b;
+    if (c & kLeafBit) {
+        uint32_t f = c & kFirstMask, cnt = ((c >> 27) & 0xfu) + 1u;
+        for (uint32_t k = f; k < f + cnt; ++k) b = box_union(b, leaf_boxes[order ? order[k] : k]);
+        return b;
+    }
+    return node_box(nodes[c]);
+}
+
+// Surface area of a box (half of it: the collapse only compares areas).
+__device__ inline float box_area(const Box& b) {
+    float ex = b.hi[0] - b.lo[0], ey = b.hi[1] - b.lo[1], ez = b.hi[2] - b.lo[2];
+    return (ex * ey + ey * ez) + ez * ex;
 }
 
 // Block-wide reduction of centre bounds, then one atomic per block and axis.
@@ -386,7 +416,7 @@ __device__ inline int delta(const uint32_t* keys, int n, int i, int j) {
 }
 
 __global__ __launch_bounds__(256) void k_karras(const uint32_t* keys, int n, uint32_t* left, uint32_t* right,
-                                                uint32_t* parent) {
+                                                uint32_t* parent, uint32_t* rfirst, uint32_t* rcount) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n - 1) return;
     int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -410,16 +440,19 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* keys, int n, uin
     uint32_t rc = (hi == gamma + 1) ? (kLeafBit | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
     left[i] = lc;
     right[i] = rc;
+    rfirst[i] = (uint32_t)lo;
+    rcount[i] = (uint32_t)(hi - lo + 1);
     // parent array: internal nodes at [0, n-1), leaves at [n-1, 2n-1)
     parent[(lc & kLeafBit) ? (n - 1) + (lc & kFirstMask) : lc] = (uint32_t)i;
     parent[(rc & kLeafBit) ? (n - 1) + (rc & kFirstMask) : rc] = (uint32_t)i;
     if (i == 0) parent[0] = 0xffffffffu;
 }
 
-// Bottom-up boxes: the second child to arrive at a node finishes it.
+// Bottom-up boxes of the binary tree (the second child to arrive at a node
+// finishes it) and their surface areas.
 __global__ __launch_bounds__(256) void k_bottom_up(const Box* boxes, const uint32_t* order, int n, const uint32_t* left,
                                                    const uint32_t* right, const uint32_t* parent, uint32_t* arrivals,
-                                                   uint32_t* height, Box* ibox, Node* nodes) {
+                                                   Box* ibox, float* area) {
     int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
     uint32_t node = parent[(n - 1) + k];
@@ -430,41 +463,121 @@ __global__ __launch_bounds__(256) void k_bottom_up(const Box* boxes, const uint3
         __threadfence();
         uint32_t c[2] = {left[node], right[node]};
         Box cbx[2];
-        uint32_t hmax = 0;
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            if (c[s] & kLeafBit) {
-                cbx[s] = boxes[order[c[s] & kFirstMask]];
-            } else {
-                cbx[s] = ibox[c[s]];
-                hmax = max(hmax, height[c[s]]);
-            }
-        }
-        Node nd;
-        set_child(nd, 0, c[0], cbx[0]);
-        set_child(nd, 1, c[1], cbx[1]);
-        nd.pad[0] = nd.pad[1] = 0;
-        nodes[node] = nd;
-        ibox[node] = box_union(cbx[0], cbx[1]);
-        height[node] = hmax + 1;
+        for (int s = 0; s < 2; ++s) cbx[s] = (c[s] & kLeafBit) ? boxes[order[c[s] & kFirstMask]] : ibox[c[s]];
+        Box u = box_union(cbx[0], cbx[1]);
+        ibox[node] = u;
+        area[node] = box_area(u);
         node = parent[node];
     }
 }
 
-// Root node of a structure with at most one primitive.
-__global__ void k_small_root(const Box* boxes, uint32_t n, Node* nodes) {
-    if (threadIdx.x != 0) return;
-    Node nd;
-    Box e = box_empty();
-    set_child(nd, 0, n == 1 ? kLeafBit : kEmpty, n == 1 ? boxes[0] : e);
-    set_child(nd, 1, kEmpty, e);
-    nd.pad[0] = nd.pad[1] = 0;
-    nodes[0] = nd;
+// The single node of a structure with at most leaf_max primitives: one leaf
+// of all of them in input order (or none).
+__global__ void k_wide_single(const Box* boxes, uint32_t n, Node* nodes, uint32_t* order) {
+    if (threadIdx.x == 0) {
+        Node nd;
+        const uint32_t ref = n ? (kLeafBit | ((n - 1) << 27)) : kEmpty;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            nd.child[s] = s == 0 ? ref : kEmpty;
+            nd.pad[s] = 0;
+            set_slot(nd, s, s == 0 ? ref_box(nullptr, ref, boxes, nullptr) : box_empty());
+        }
+        nodes[0] = nd;
+    }
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) order[k] = k;
+}
+
+// Slots (binary refs) of each wide node of a level: the binary root's two
+// children, then the largest-area openable slot replaced in place by its
+// two children while fewer than four; and the number of slots that become
+// wide nodes of the next level.
+__global__ __launch_bounds__(256) void k_wide_expand(uint32_t lb, uint32_t m, const uint32_t* wroot,
+                                                     const uint32_t* left, const uint32_t* right,
+                                                     const uint32_t* rcount, const float* area, uint32_t leaf_max,
+                                                     uint4* slots, uint32_t* cnt) {
+    uint32_t w = blockIdx.x * 256 + threadIdx.x;
+    if (w >= m) return;
+    uint32_t r = wroot[lb + w];
+    uint32_t s[4] = {left[r], right[r], kEmpty, kEmpty};
+    int nslot = 2;
+    while (nslot < 4) {
+        int best = -1;
+        float best_area = 0.0f;
+        for (int k = 0; k < nslot; ++k) {
+            uint32_t c = s[k];
+            if ((c & kLeafBit) || rcount[c] <= leaf_max) continue;
+            float a = area[c];
+            if (best < 0 || a > best_area) {
+                best = k;
+                best_area = a;
+            }
+        }
+        if (best < 0) break;
+        uint32_t c = s[best];
+        for (int k = nslot; k > best + 1; --k) s[k] = s[k - 1];
+        s[best] = left[c];
+        s[best + 1] = right[c];
+        ++nslot;
+    }
+    uint32_t internal = 0;
+    for (int k = 0; k < nslot; ++k) internal += (!(s[k] & kLeafBit) && rcount[s[k]] > leaf_max) ? 1u : 0u;
+    slots[w] = make_uint4(s[0], s[1], s[2], s[3]);
+    cnt[w] = internal;
+}
+
+// Child refs of each wide node of a level; opened slots get the next
+// level's node numbers from the exclusive prefix sum `base` (node le + base).
+__global__ __launch_bounds__(256) void k_wide_emit(uint32_t lb, uint32_t le, uint32_t m, const uint4* slots,
+                                                   const uint32_t* base, const uint32_t* rfirst,
+                                                   const uint32_t* rcount, uint32_t leaf_max, uint32_t cap,
+                                                   Node* nodes, uint32_t* wroot, uint32_t* next_count) {
+    uint32_t w = blockIdx.x * 256 + threadIdx.x;
+    if (w >= m) return;
+    uint4 sv = slots[w];
+    uint32_t s[4] = {sv.x, sv.y, sv.z, sv.w};
+    uint32_t next = le + base[w];
+    uint32_t internal = 0;
+    Node& nd = nodes[lb + w];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint32_t c = s[k], ref = kEmpty;
+        if (c != kEmpty) {
+            if (c & kLeafBit) {
+                ref = c;
+            } else if (rcount[c] <= leaf_max) {
+                ref = kLeafBit | ((rcount[c] - 1u) << 27) | rfirst[c];
+            } else {
+                // a wide tree over n leaves has at most n - 1 nodes (each has
+                // two or more slots); the host checks the count against cap
+                ref = next;
+                if (next < cap) wroot[next] = c;
+                ++next;
+                ++internal;
+            }
+        }
+        nd.child[k] = ref;
+    }
+    if (w == m - 1) *next_count = base[w] + internal;
+}
+
+// Slot boxes of the wide nodes of a level (the levels below are done).
+__global__ __launch_bounds__(256) void k_wide_boxes(uint32_t lb, uint32_t m, Node* nodes, const Box* boxes,
+                                                    const uint32_t* order) {
+    uint32_t w = blockIdx.x * 256 + threadIdx.x;
+    if (w >= m) return;
+    Node& nd = nodes[lb + w];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        set_slot(nd, k, ref_box(nodes, nd.child[k], boxes, order));
+        nd.pad[k] = 0;
+    }
 }
 
 __global__ void k_header(AsHeader* h, uint32_t kind, uint32_t node_count, uint32_t prim_count, uint32_t geom_count,
-                         const uint32_t* height, uint32_t n, uint64_t nodes_off, uint64_t prims_off,
-                         uint64_t extra_off, uint64_t level_off, uint64_t leafbox_off, uint64_t total) {
+                         uint32_t levels, uint64_t nodes_off, uint64_t prims_off, uint64_t extra_off,
+                         uint64_t level_off, uint64_t leafbox_off, uint64_t total) {
     if (threadIdx.x != 0) return;
     AsHeader hd;
     memset(&hd, 0, sizeof(hd));
@@ -473,8 +586,8 @@ __global__ void k_header(AsHeader* h, uint32_t kind, uint32_t node_count, uint32
     hd.node_count = node_count;
     hd.prim_count = prim_count;
     hd.geom_count = geom_count;
-    hd.max_depth = n >= 2 ? height[0] : 1u;
-    hd.node_width = 2;
+    hd.max_depth = levels;
+    hd.node_width = 4;
     hd.flags = 0;
     const Node* nodes = reinterpret_cast<const Node*>(reinterpret_cast<const uint8_t*>(h) + nodes_off);
     Box r = node_box(nodes[0]);
@@ -506,38 +619,6 @@ __global__ __launch_bounds__(256) void k_gather_insts(const uint32_t* order, con
     }
 }
 
-// Level lists of the internal nodes, deepest level first, ascending index
-// within a level (one block; used once per TLAS build).
-__global__ __launch_bounds__(1024) void k_tlas_levels(const uint32_t* parent, uint32_t nint, uint32_t* depth,
-                                                      uint32_t* level_data) {
-    __shared__ uint32_t s_max;
-    if (threadIdx.x == 0) s_max = 0;
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < nint; i += blockDim.x) {
-        uint32_t dpt = 0;
-        for (uint32_t p = parent[i]; p != 0xffffffffu; p = parent[p]) ++dpt;
-        depth[i] = dpt;
-        atomicMax(&s_max, dpt);
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    uint32_t levels = nint ? s_max + 1 : 0;
-    level_data[0] = levels;
-    uint32_t* offs = level_data + 1;
-    uint32_t* list = level_data + 2 + levels;
-    // counts per level (deepest level first)
-    for (uint32_t L = 0; L <= levels; ++L) offs[L] = 0;
-    for (uint32_t i = 0; i < nint; ++i) offs[levels - 1 - depth[i] + 1]++;
-    for (uint32_t L = 0; L < levels; ++L) offs[L + 1] += offs[L];
-    // fill in ascending node order; offs[L] is used as a cursor then restored
-    for (uint32_t i = 0; i < nint; ++i) {
-        uint32_t L = levels - 1 - depth[i];
-        list[offs[L]++] = i;
-    }
-    for (uint32_t L = levels; L > 0; --L) offs[L] = offs[L - 1];
-    offs[0] = 0;
-}
-
 // In-place TLAS UPDATE (src/util_structs.rs:285-357): new instance records,
 // same topology. One workgroup walks the level lists from the deepest level
 // to the root with a barrier between levels.
@@ -559,27 +640,18 @@ __global__ __launch_bounds__(1024) void k_tlas_refit(uint8_t* base, const rtg_in
         src[s] = in;
     }
     __syncthreads();
-    if (n >= 2) {
-        const uint32_t* lv = reinterpret_cast<const uint32_t*>(base + h->level_off);
-        uint32_t levels = lv[0];
-        const uint32_t* offs = lv + 1;
-        const uint32_t* list = lv + 2 + levels;
-        for (uint32_t L = 0; L < levels; ++L) {
-            for (uint32_t q = offs[L] + threadIdx.x; q < offs[L + 1]; q += blockDim.x) {
-                Node& nd = nodes[list[q]];
+    // same topology: slot boxes again, deepest level first
+    const uint32_t* lv = reinterpret_cast<const uint32_t*>(base + h->level_off);
+    const uint32_t levels = lv[0];
+    const uint32_t* offs = lv + 1;
+    for (uint32_t L = levels; L-- > 0;) {
+        for (uint32_t q = offs[L] + threadIdx.x; q < offs[L + 1]; q += blockDim.x) {
+            Node& nd = nodes[q];
 #pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                    uint32_t c = nd.child[s];
-                    Box cb = (c & kLeafBit) ? leafbox[c & kFirstMask] : node_box(nodes[c]);
-                    set_child(nd, s, c, cb);
-                }
-            }
-            __syncthreads();
+            for (int s = 0; s < 4; ++s) set_slot(nd, s, ref_box(nodes, nd.child[s], leafbox, nullptr));
         }
-    } else if (n == 1 && threadIdx.x == 0) {
-        set_child(nodes[0], 0, kLeafBit, leafbox[0]);
+        __syncthreads();
     }
-    __syncthreads();
     if (threadIdx.x == 0) {
         Box r = node_box(nodes[0]);
         for (int a = 0; a < 3; ++a) {
@@ -600,11 +672,12 @@ struct Layout {
     uint32_t node_count;
 };
 
-static Layout blas_layout(uint32_t n, uint32_t ngeom) {
+// Blob layouts for a tree of node_count nodes in `levels` levels.
+static Layout blas_layout(uint32_t n, uint32_t ngeom, uint32_t node_count) {
     Layout L;
-    L.node_count = n >= 2 ? n - 1 : 1;
+    L.node_count = node_count;
     L.nodes_off = 128;
-    L.prims_off = align64(L.nodes_off + (uint64_t)L.node_count * 64);
+    L.prims_off = align64(L.nodes_off + (uint64_t)node_count * sizeof(Node));
     L.extra_off = align64(L.prims_off + (uint64_t)n * 48);
     L.level_off = 0;
     L.leafbox_off = 0;
@@ -612,23 +685,27 @@ static Layout blas_layout(uint32_t n, uint32_t ngeom) {
     return L;
 }
 
-static Layout tlas_layout(uint32_t n) {
+static Layout tlas_layout(uint32_t n, uint32_t node_count, uint32_t levels) {
     Layout L;
-    uint32_t nint = n >= 2 ? n - 1 : 0;
-    L.node_count = n >= 2 ? n - 1 : 1;
+    L.node_count = node_count;
     L.nodes_off = 128;
-    L.prims_off = align64(L.nodes_off + (uint64_t)L.node_count * 64);
+    L.prims_off = align64(L.nodes_off + (uint64_t)node_count * sizeof(Node));
     L.extra_off = align64(L.prims_off + (uint64_t)n * 64);
     L.level_off = align64(L.extra_off + (uint64_t)n * 64);
-    L.leafbox_off = align64(L.level_off + 4ull * (1 + (nint + 1) + nint));
+    L.leafbox_off = align64(L.level_off + 4ull * (2 + levels));
     L.total = align64(L.leafbox_off + (uint64_t)n * 24);
     return L;
 }
 
-// Scratch needs of the generic LBVH over n primitives.
+// Node capacity of a wide tree over n primitives (its node count is at most
+// n - 1: every node has two or more slots).
+static inline uint32_t node_cap(uint32_t n) { return n >= 2 ? n - 1 : 1; }
+
+// Scratch needs of the wide-tree build over n primitives.
 static size_t lbvh_scratch_bytes(uint32_t n, size_t rec_bytes) {
     size_t off = 0;
     uint32_t ntiles = (n + kRsTile - 1) / kRsTile;
+    uint32_t cap = node_cap(n);
     carve_size<uint8_t>(off, (size_t)n * rec_bytes);  // unsorted records
     carve_size<Box>(off, n);                          // primitive boxes
     carve_size<uint32_t>(off, 8);                     // centre bounds
@@ -641,9 +718,15 @@ static size_t lbvh_scratch_bytes(uint32_t n, size_t rec_bytes) {
     carve_size<uint32_t>(off, n);                     // right
     carve_size<uint32_t>(off, 2ull * n);              // parent
     carve_size<uint32_t>(off, n);                     // arrivals
-    carve_size<uint32_t>(off, n);                     // height
-    carve_size<Box>(off, n);                          // internal boxes
-    carve_size<uint32_t>(off, n);                     // depth
+    carve_size<uint32_t>(off, n);                     // range first
+    carve_size<uint32_t>(off, n);                     // range count
+    carve_size<float>(off, n);                        // binary box areas
+    carve_size<Box>(off, n);                          // binary boxes
+    carve_size<Node>(off, cap);                       // wide nodes
+    carve_size<uint32_t>(off, cap);                   // binary root of each wide node
+    carve_size<uint4>(off, cap);                      // slots of a level
+    carve_size<uint32_t>(off, cap);                   // opened slots per node / prefix sums
+    carve_size<uint32_t>(off, 4);                     // next level size
     carve_size<GeomDesc>(off, 1);
     return off + 4096;
 }
@@ -652,8 +735,12 @@ struct LbvhBufs {
     uint8_t* recs;
     Box* boxes;
     uint32_t* cb;
-    uint32_t *ka, *kb, *va, *vb, *hist, *left, *right, *parent, *arrivals, *height, *depth;
+    uint32_t *ka, *kb, *va, *vb, *hist, *left, *right, *parent, *arrivals, *rfirst, *rcount;
+    float* area;
     Box* ibox;
+    Node* nodes;
+    uint32_t *wroot, *cnt, *next;
+    uint4* slots;
     GeomDesc* geoms;
 };
 
@@ -662,6 +749,7 @@ static LbvhBufs carve_lbvh(void* base, uint32_t n, size_t rec_bytes, uint32_t ng
     LbvhBufs b;
     b.geoms = c.take<GeomDesc>(std::max(1u, ngeom));
     uint32_t ntiles = (n + kRsTile - 1) / kRsTile;
+    uint32_t cap = node_cap(n);
     b.recs = c.take<uint8_t>((size_t)n * rec_bytes);
     b.boxes = c.take<Box>(n);
     b.cb = c.take<uint32_t>(8);
@@ -674,19 +762,29 @@ static LbvhBufs carve_lbvh(void* base, uint32_t n, size_t rec_bytes, uint32_t ng
     b.right = c.take<uint32_t>(n);
     b.parent = c.take<uint32_t>(2ull * n);
     b.arrivals = c.take<uint32_t>(n);
-    b.height = c.take<uint32_t>(n);
+    b.rfirst = c.take<uint32_t>(n);
+    b.rcount = c.take<uint32_t>(n);
+    b.area = c.take<float>(n);
     b.ibox = c.take<Box>(n);
-    b.depth = c.take<uint32_t>(n);
+    b.nodes = c.take<Node>(cap);
+    b.wroot = c.take<uint32_t>(cap);
+    b.slots = c.take<uint4>(cap);
+    b.cnt = c.take<uint32_t>(cap);
+    b.next = c.take<uint32_t>(4);
     return b;
 }
 
-// Sort keys/values, build the radix tree and its boxes into `nodes`.
-// On return the sorted primitive order is in b.va.
-static int lbvh_hierarchy(LbvhBufs& b, uint32_t n, Node* nodes, hipStream_t s) {
-    if (n <= 1) {
-        hipLaunchKernelGGL(k_small_root, dim3(1), dim3(64), 0, s, b.boxes, n, nodes);
-        if (n == 1) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, b.va, 1ull, 0u);
+// Sorts the primitives, builds the binary radix tree and collapses it into
+// the wide tree in b.nodes. On return the leaf order is in b.va and
+// level_off holds the level offsets (level_off.back() = node count). Reads
+// each level's size back from the device (synchronises `s`).
+static int wide_hierarchy(LbvhBufs& b, uint32_t n, uint32_t leaf_max, hipStream_t s,
+                          std::vector<uint32_t>& level_off) {
+    level_off.assign(1, 0u);
+    if (n <= leaf_max || n <= 1) {
+        hipLaunchKernelGGL(k_wide_single, dim3(1), dim3(256), 0, s, b.boxes, n, b.nodes, b.va);
         RTG_HIP(hipGetLastError());
+        level_off.push_back(1u);
         return RTG_OK;
     }
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.boxes, n, b.cb, b.ka, b.va);
@@ -703,10 +801,37 @@ static int lbvh_hierarchy(LbvhBufs& b, uint32_t n, Node* nodes, hipStream_t s) {
     }
     // after 4 passes the sorted data is back in (ka, va)
     hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, b.ka, (int)n, b.left, b.right,
-                       b.parent);
+                       b.parent, b.rfirst, b.rcount);
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.arrivals, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_bottom_up, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.boxes, b.va, (int)n, b.left, b.right,
-                       b.parent, b.arrivals, b.height, b.ibox, nodes);
+                       b.parent, b.arrivals, b.ibox, b.area);
+    // top-down collapse, one breadth-first level per step
+    const uint32_t cap = node_cap(n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, b.wroot, 1ull, 0u);
+    uint32_t lb = 0, le = 1;
+    while (lb < le) {
+        const uint32_t m = le - lb;
+        hipLaunchKernelGGL(k_wide_expand, dim3(blocks_for(m, 256)), dim3(256), 0, s, lb, m, b.wroot, b.left, b.right,
+                           b.rcount, b.area, leaf_max, b.slots, b.cnt);
+        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, b.cnt, m);
+        hipLaunchKernelGGL(k_wide_emit, dim3(blocks_for(m, 256)), dim3(256), 0, s, lb, le, m, b.slots, b.cnt,
+                           b.rfirst, b.rcount, leaf_max, cap, b.nodes, b.wroot, b.next);
+        RTG_HIP(hipGetLastError());
+        uint32_t next = 0;
+        RTG_HIP(hipMemcpyAsync(&next, b.next, 4, hipMemcpyDeviceToHost, s));
+        RTG_HIP(hipStreamSynchronize(s));
+        RTG_REQUIRE(next <= cap - le, RTG_ERROR_DEVICE, "BVH collapse: %u nodes exceed the capacity %u", le + next,
+                    cap);
+        level_off.push_back(le);
+        lb = le;
+        le += next;
+    }
+    const uint32_t levels = (uint32_t)level_off.size() - 1;
+    for (uint32_t L = levels; L-- > 0;) {
+        const uint32_t m = level_off[L + 1] - level_off[L];
+        hipLaunchKernelGGL(k_wide_boxes, dim3(blocks_for(m, 256)), dim3(256), 0, s, level_off[L], m, b.nodes,
+                           b.boxes, b.va);
+    }
     RTG_HIP(hipGetLastError());
     return RTG_OK;
 }
@@ -759,7 +884,8 @@ extern "C" int rtg_blas_query_sizes(rtg_ctx* ctx, const rtg_geometry* geoms, uin
     for (uint32_t g = 0; g < n_geom; ++g) total += geoms[g].index_count / 3;
     RTG_REQUIRE(total < (1ull << 27), RTG_ERROR_LIMIT, "BLAS has %llu triangles (limit 2^27)",
                 (unsigned long long)total);
-    Layout L = blas_layout((uint32_t)total, n_geom);
+    // an upper bound: the blob of a build holds its actual node count
+    Layout L = blas_layout((uint32_t)total, n_geom, node_cap((uint32_t)total));
     out->acceleration_structure_size = L.total;
     out->build_scratch_size = lbvh_scratch_bytes((uint32_t)total, sizeof(TriRec)) + n_geom * sizeof(GeomDesc);
     out->update_scratch_size = 0;
@@ -770,7 +896,7 @@ extern "C" int rtg_tlas_query_sizes(rtg_ctx* ctx, uint32_t count, rtg_build_size
     clear_error();
     RTG_REQUIRE(ctx && out, RTG_ERROR_INVALID_ARGUMENT, "rtg_tlas_query_sizes: null argument");
     RTG_REQUIRE(count < (1u << 27), RTG_ERROR_LIMIT, "TLAS has %u instances (limit 2^27)", count);
-    Layout L = tlas_layout(count);
+    Layout L = tlas_layout(count, node_cap(count), node_cap(count));
     out->acceleration_structure_size = L.total;
     out->build_scratch_size = lbvh_scratch_bytes(count, sizeof(InstRec));
     out->update_scratch_size = 0;
@@ -803,15 +929,8 @@ extern "C" int rtg_blas_build(rtg_ctx* ctx, const rtg_geometry* geoms, uint32_t 
         RTG_REQUIRE(total < (1ull << 27), RTG_ERROR_LIMIT, "BLAS exceeds 2^27 triangles");
     }
     uint32_t n = (uint32_t)total;
-    Layout L = blas_layout(n, n_geom);
-    uint8_t* blob = nullptr;
-    RTG_HIP(hipMalloc(&blob, L.total));
-    RTG_HIP(hipMemsetAsync(blob, 0, L.total, s));
     size_t need = lbvh_scratch_bytes(n, sizeof(TriRec)) + std::max(1u, n_geom) * sizeof(GeomDesc) + 256;
-    if (ctx->scratch.ensure(need) != RTG_OK) {
-        hipFree(blob);
-        return RTG_ERROR_OUT_OF_MEMORY;
-    }
+    if (ctx->scratch.ensure(need) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
     LbvhBufs b = carve_lbvh(ctx->scratch.ptr, n, sizeof(TriRec), n_geom);
     GeomDesc* d_geoms = b.geoms;
     // the copy source must outlive the asynchronous copy: the build
@@ -821,11 +940,24 @@ extern "C" int rtg_blas_build(rtg_ctx* ctx, const rtg_geometry* geoms, uint32_t 
     if (n > 0)
         hipLaunchKernelGGL(k_tri_setup, dim3(blocks_for(n, 256)), dim3(256), 0, s, d_geoms, n_geom, n,
                            reinterpret_cast<TriRec*>(b.recs), b.boxes, b.cb, ctx->d_status);
-    Node* nodes = reinterpret_cast<Node*>(blob + L.nodes_off);
-    int st = lbvh_hierarchy(b, n, nodes, s);
+    std::vector<uint32_t> level_off;
+    int st = wide_hierarchy(b, n, kBlasLeafMax, s, level_off);
     if (st != RTG_OK) {
-        hipFree(blob);
+        hipStreamSynchronize(s);  // pending copies read host memory of this frame
         return st;
+    }
+    const uint32_t levels = (uint32_t)level_off.size() - 1;
+    Layout L = blas_layout(n, n_geom, level_off.back());
+    uint8_t* blob = nullptr;
+    RTG_HIP(hipMalloc(&blob, L.total));
+    hipError_t e = hipMemsetAsync(blob, 0, L.total, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(blob + L.nodes_off, b.nodes, (size_t)L.node_count * sizeof(Node), hipMemcpyDeviceToDevice,
+                           s);
+    if (e != hipSuccess) {
+        hipFree(blob);
+        set_error("rtg_blas_build: %s", hipGetErrorString(e));
+        return RTG_ERROR_DEVICE;
     }
     if (n > 0)
         hipLaunchKernelGGL(k_gather_tris, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.va,
@@ -834,8 +966,7 @@ extern "C" int rtg_blas_build(rtg_ctx* ctx, const rtg_geometry* geoms, uint32_t 
     for (uint32_t g = 0; g < n_geom; ++g) gflags[g] = geoms[g].flags;
     if (n_geom) RTG_HIP(hipMemcpyAsync(blob + L.extra_off, gflags.data(), n_geom * 4, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_header, dim3(1), dim3(64), 0, s, reinterpret_cast<AsHeader*>(blob), kKindBlas, L.node_count,
-                       n, n_geom, b.height, n, L.nodes_off, L.prims_off, L.extra_off, L.level_off, L.leafbox_off,
-                       L.total);
+                       n, n_geom, levels, L.nodes_off, L.prims_off, L.extra_off, L.level_off, L.leafbox_off, L.total);
     RTG_HIP(hipGetLastError());
     st = check_status(ctx, s, kStatusBadIndex, "rtg_blas_build");
     if (st != RTG_OK) {
@@ -859,39 +990,49 @@ extern "C" int rtg_tlas_build(rtg_ctx* ctx, const rtg_instance* instances, uint3
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     RTG_HIP(hipSetDevice(ctx->device));
     uint32_t n = count;
-    Layout L = tlas_layout(n);
-    uint8_t* blob = nullptr;
-    RTG_HIP(hipMalloc(&blob, L.total));
-    RTG_HIP(hipMemsetAsync(blob, 0, L.total, s));
     size_t need = lbvh_scratch_bytes(n, sizeof(InstRec));
-    if (ctx->scratch.ensure(need) != RTG_OK) {
-        hipFree(blob);
-        return RTG_ERROR_OUT_OF_MEMORY;
-    }
+    if (ctx->scratch.ensure(need) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
     LbvhBufs b = carve_lbvh(ctx->scratch.ptr, n, sizeof(InstRec));
     hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, s, b.cb);
     if (n > 0)
         hipLaunchKernelGGL(k_inst_setup, dim3(blocks_for(n, 256)), dim3(256), 0, s, instances, n, ctx->d_registry,
                            ctx->registry_count, reinterpret_cast<InstRec*>(b.recs), b.boxes, b.cb, ctx->d_status);
-    Node* nodes = reinterpret_cast<Node*>(blob + L.nodes_off);
-    int st = lbvh_hierarchy(b, n, nodes, s);
+    std::vector<uint32_t> level_off;
+    int st = wide_hierarchy(b, n, kTlasLeafMax, s, level_off);
     if (st != RTG_OK) {
-        hipFree(blob);
+        hipStreamSynchronize(s);
         return st;
     }
-    if (n > 0) {
+    const uint32_t levels = (uint32_t)level_off.size() - 1;
+    Layout L = tlas_layout(n, level_off.back(), levels);
+    // level data: level_count, level_offsets[level_count + 1]
+    std::vector<uint32_t> level_data(1, levels);
+    level_data.insert(level_data.end(), level_off.begin(), level_off.end());
+    uint8_t* blob = nullptr;
+    RTG_HIP(hipMalloc(&blob, L.total));
+    hipError_t e = hipMemsetAsync(blob, 0, L.total, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(blob + L.nodes_off, b.nodes, (size_t)L.node_count * sizeof(Node), hipMemcpyDeviceToDevice,
+                           s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(blob + L.level_off, level_data.data(), level_data.size() * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && n > 0)
+        e = hipMemcpyAsync(blob + L.extra_off, instances, (size_t)n * 64, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) {
+        hipStreamSynchronize(s);
+        hipFree(blob);
+        set_error("rtg_tlas_build: %s", hipGetErrorString(e));
+        return RTG_ERROR_DEVICE;
+    }
+    if (n > 0)
         hipLaunchKernelGGL(k_gather_insts, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.va,
                            reinterpret_cast<const InstRec*>(b.recs), b.boxes,
                            reinterpret_cast<InstRec*>(blob + L.prims_off), reinterpret_cast<Box*>(blob + L.leafbox_off),
                            n);
-        RTG_HIP(hipMemcpyAsync(blob + L.extra_off, instances, (size_t)n * 64, hipMemcpyDeviceToDevice, s));
-    }
-    uint32_t nint = n >= 2 ? n - 1 : 0;
-    hipLaunchKernelGGL(k_tlas_levels, dim3(1), dim3(1024), 0, s, b.parent, nint, b.depth,
-                       reinterpret_cast<uint32_t*>(blob + L.level_off));
     hipLaunchKernelGGL(k_header, dim3(1), dim3(64), 0, s, reinterpret_cast<AsHeader*>(blob), kKindTlas, L.node_count,
-                       n, 0u, b.height, n, L.nodes_off, L.prims_off, L.extra_off, L.level_off, L.leafbox_off, L.total);
+                       n, 0u, levels, L.nodes_off, L.prims_off, L.extra_off, L.level_off, L.leafbox_off, L.total);
     RTG_HIP(hipGetLastError());
+    // also keeps level_data alive until its copy has finished
     st = check_status(ctx, s, kStatusBadBlas, "rtg_tlas_build");
     if (st != RTG_OK) {
         hipFree(blob);

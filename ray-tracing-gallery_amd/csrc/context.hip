@@ -172,6 +172,16 @@ extern "C" int rtg_get_counters(rtg_ctx* ctx, rtg_trace_counters* out) {
     return RTG_OK;
 }
 
+extern "C" int rtg_get_status(rtg_ctx* ctx, uint32_t* bits, int clear) {
+    clear_error();
+    RTG_REQUIRE(ctx && bits, RTG_ERROR_INVALID_ARGUMENT, "rtg_get_status: null argument");
+    RTG_HIP(hipSetDevice(ctx->device));
+    RTG_HIP(hipDeviceSynchronize());
+    RTG_HIP(hipMemcpy(bits, ctx->d_status, 4, hipMemcpyDeviceToHost));
+    if (clear) RTG_HIP(hipMemset(ctx->d_status, 0, 4));
+    return RTG_OK;
+}
+
 extern "C" int rtg_set_timing(rtg_ctx* ctx, int enable) {
     clear_error();
     RTG_REQUIRE(ctx, RTG_ERROR_INVALID_ARGUMENT, "rtg_set_timing: null context");

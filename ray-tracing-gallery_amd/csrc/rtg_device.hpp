@@ -16,18 +16,28 @@ namespace rtg {
 
 // ---------------------------------------------------------------------------
 // Acceleration-structure blob layout (one device allocation per structure).
-//   header (128 B) | nodes (64 B) | prims | extra [| level data | leaf boxes]
+//   header (128 B) | nodes (128 B, breadth first) | prims | extra
+//   [| level data | leaf boxes]
 // BLAS prims: 48-B triangle records in leaf order
 //   {v0.xyz, primitive id} {v1.xyz, geometry id | non-opaque bit 31} {v2.xyz, 0}
 // TLAS prims: 64-B instance leaf records in leaf order
 //   {world-to-object 3x4 row-major} {BLAS address, instance index, mask | flags<<8}
 // TLAS extra: copy of the source rtg_instance records (original order).
-// TLAS level data: level_count, level_offsets[level_count+1], level_nodes[]
-//   (internal nodes, deepest level first, ascending index within a level).
+// TLAS level data: level_count, level_offsets[level_count+1] (level L is
+//   nodes [off[L], off[L+1]) of the breadth-first order, root level 0).
 // TLAS leaf boxes: world box of each leaf (24 B), scratch of the refit.
-// Node: x planes (lo0, hi0, lo1, hi1), y planes, z planes, child refs.
+// Node: four child slots as lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4],
+//   hi_z[4], child refs[4], 4 zero words; unused slots are EMPTY with the
+//   empty box. A slot box is the fold, from the empty box, of its leaf's
+//   primitive boxes in leaf order or of its child node's four slot boxes.
 // Child ref: internal node index, or LEAF_BIT | (count-1) << 27 | first,
 //   or EMPTY.
+// The tree is the binary radix tree of the build (Karras 2012) collapsed
+// top-down into 4-wide nodes: a node rooted at binary node r takes r's two
+// children and, while it has fewer than four slots, opens the slot whose
+// subtree box has the largest surface area (first slot on ties) in place;
+// subtrees of at most kBlasLeafMax (BLAS) or kTlasLeafMax (TLAS) primitives
+// are leaves and are not opened.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kMagic = 0x53415452u;  // 'RTAS'
 constexpr uint32_t kKindBlas = 0u;
@@ -40,6 +50,8 @@ constexpr uint32_t kNonOpaqueBit = 0x80000000u;
 // 1 + 2*gamma(3) rounded up: far-plane inflation of the robust slab test
 // (Ize 2013, "Robust BVH Ray Traversal", JCGT 2(2)).
 constexpr uint32_t kFarScaleBits = 0x3f800003u;
+constexpr uint32_t kBlasLeafMax = 1u;
+constexpr uint32_t kTlasLeafMax = 1u;
 
 struct AsHeader {
     uint32_t magic, kind, node_count, prim_count;
@@ -56,13 +68,13 @@ struct AsHeader {
 static_assert(sizeof(AsHeader) == 128, "header");
 
 struct Node {
-    float x[4];
-    float y[4];
-    float z[4];
-    uint32_t child[2];
-    uint32_t pad[2];
+    float lo_x[4], hi_x[4];
+    float lo_y[4], hi_y[4];
+    float lo_z[4], hi_z[4];
+    uint32_t child[4];
+    uint32_t pad[4];
 };
-static_assert(sizeof(Node) == 64, "node");
+static_assert(sizeof(Node) == 128, "node");
 
 struct TriRec {
     float v0[3];
@@ -411,36 +423,21 @@ __device__ inline bool tri_test(const RayCtx& r, V3 p0, V3 p1, V3 p2, float tmin
     return true;
 }
 
-// Robust slab test of both children of a node (Ize 2013). Near and far planes
-// are chosen by the sign bit of the ray direction.
-__device__ inline void box_test2(const float4& X, const float4& Y, const float4& Z, const RayCtx& r, float tmin,
-                                 float tcur, bool& h0, bool& h1, float& tn0, float& tn1) {
-    const bool sx = signbit(r.d.x), sy = signbit(r.d.y), sz = signbit(r.d.z);
-    const float far_scale = u2f(kFarScaleBits);
-    {
-        float nx = sx ? X.y : X.x, fx = sx ? X.x : X.y;
-        float ny = sy ? Y.y : Y.x, fy = sy ? Y.x : Y.y;
-        float nz = sz ? Z.y : Z.x, fz = sz ? Z.x : Z.y;
-        float tnx = (nx - r.o.x) * r.inv.x, tfx = (fx - r.o.x) * r.inv.x;
-        float tny = (ny - r.o.y) * r.inv.y, tfy = (fy - r.o.y) * r.inv.y;
-        float tnz = (nz - r.o.z) * r.inv.z, tfz = (fz - r.o.z) * r.inv.z;
-        float tnear = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
-        float tfar = fminf(fminf(fminf(tfx, tfy), tfz) * far_scale, tcur);
-        tn0 = tnear;
-        h0 = tnear <= tfar;
-    }
-    {
-        float nx = sx ? X.w : X.z, fx = sx ? X.z : X.w;
-        float ny = sy ? Y.w : Y.z, fy = sy ? Y.z : Y.w;
-        float nz = sz ? Z.w : Z.z, fz = sz ? Z.z : Z.w;
-        float tnx = (nx - r.o.x) * r.inv.x, tfx = (fx - r.o.x) * r.inv.x;
-        float tny = (ny - r.o.y) * r.inv.y, tfy = (fy - r.o.y) * r.inv.y;
-        float tnz = (nz - r.o.z) * r.inv.z, tfz = (fz - r.o.z) * r.inv.z;
-        float tnear = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
-        float tfar = fminf(fminf(fminf(tfx, tfy), tfz) * far_scale, tcur);
-        tn1 = tnear;
-        h1 = tnear <= tfar;
-    }
+// Robust slab test of one slot (Ize 2013): near and far planes chosen by
+// the sign bit of the ray direction; the far distance is inflated by
+// 1 + 2 gamma(3) before the comparison.
+__device__ inline bool slab(float lx, float hx, float ly, float hy, float lz, float hz, const RayCtx& r, bool sx,
+                            bool sy, bool sz, float tmin, float tcur, float& tn) {
+    float nx = sx ? hx : lx, fx = sx ? lx : hx;
+    float ny = sy ? hy : ly, fy = sy ? ly : hy;
+    float nz = sz ? hz : lz, fz = sz ? lz : hz;
+    float tnx = (nx - r.o.x) * r.inv.x, tfx = (fx - r.o.x) * r.inv.x;
+    float tny = (ny - r.o.y) * r.inv.y, tfy = (fy - r.o.y) * r.inv.y;
+    float tnz = (nz - r.o.z) * r.inv.z, tfz = (fz - r.o.z) * r.inv.z;
+    float tnear = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
+    float tfar = fminf(fminf(fminf(tfx, tfy), tfz) * u2f(kFarScaleBits), tcur);
+    tn = tnear;
+    return tnear <= tfar;
 }
 
 // ---------------------------------------------------------------------------
@@ -511,7 +508,7 @@ __device__ inline bool alpha_clip_keep(const SceneView& S, const rtg_instance* s
 // (entry-major so that lanes never conflict), deeper entries in a per-thread
 // global spill area of kSpillDepth entries.
 constexpr int kLdsDepth = 16;
-constexpr int kSpillDepth = 48;
+constexpr int kSpillDepth = 80;
 constexpr int kStackMax = kLdsDepth + kSpillDepth;
 
 struct Stack {
@@ -601,26 +598,37 @@ struct Traverser {
         if (!(cur & kLeafBit)) {
             const RayCtx& cr = in_blas ? obj : world;
             const float4* np = reinterpret_cast<const float4*>(nodes + cur);
-            float4 X = gld_f4(np), Y = gld_f4(np + 1), Z = gld_f4(np + 2);
-            uint4 C = gld_u4(np + 3);
+            float4 LX = gld_f4(np), HX = gld_f4(np + 1), LY = gld_f4(np + 2), HY = gld_f4(np + 3);
+            float4 LZ = gld_f4(np + 4), HZ = gld_f4(np + 5);
+            uint4 C = gld_u4(np + 6);
             if (COUNT) cnt.nodes++;
-            bool h0, h1;
-            float tn0, tn1;
-            box_test2(X, Y, Z, cr, tmin, tcur, h0, h1, tn0, tn1);
-            h0 = h0 && C.x != kEmpty;
-            h1 = h1 && C.y != kEmpty;
-            if (h0 && h1) {
-                bool first1 = tn1 < tn0;
-                cur = first1 ? C.y : C.x;
-                if (st.sp >= kStackMax) return kRayOverflow;
-                st.push(first1 ? C.x : C.y, first1 ? tn0 : tn1);
-            } else if (h0) {
-                cur = C.x;
-            } else if (h1) {
-                cur = C.y;
-            } else if (!pop(T, st)) {
-                return done();
+            const bool sx = signbit(cr.d.x), sy = signbit(cr.d.y), sz = signbit(cr.d.z);
+            float t0, t1, t2, t3;
+            bool h0 = slab(LX.x, HX.x, LY.x, HY.x, LZ.x, HZ.x, cr, sx, sy, sz, tmin, tcur, t0) && C.x != kEmpty;
+            bool h1 = slab(LX.y, HX.y, LY.y, HY.y, LZ.y, HZ.y, cr, sx, sy, sz, tmin, tcur, t1) && C.y != kEmpty;
+            bool h2 = slab(LX.z, HX.z, LY.z, HY.z, LZ.z, HZ.z, cr, sx, sy, sz, tmin, tcur, t2) && C.z != kEmpty;
+            bool h3 = slab(LX.w, HX.w, LY.w, HY.w, LZ.w, HZ.w, cr, sx, sy, sz, tmin, tcur, t3) && C.w != kEmpty;
+            const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+            if (nh == 0) {
+                if (!pop(T, st)) return done();
+                return kRayActive;
             }
+            // rank of each overlapped slot in ascending (entry distance, slot)
+            // order; the rank-0 slot is visited next, ranks nh-1 .. 1 are
+            // pushed. A rank held by none of slots 0-2 is slot 3's.
+            const int r0 = (h1 && t1 < t0) + (h2 && t2 < t0) + (h3 && t3 < t0);
+            const int r1 = (h0 && t0 <= t1) + (h2 && t2 < t1) + (h3 && t3 < t1);
+            const int r2 = (h0 && t0 <= t2) + (h1 && t1 <= t2) + (h3 && t3 < t2);
+            if (st.sp + nh - 1 > kStackMax) return kRayOverflow;
+#pragma unroll
+            for (int q = 3; q >= 1; --q) {
+                if (q < nh) {
+                    uint32_t ref = (h0 && r0 == q) ? C.x : (h1 && r1 == q) ? C.y : (h2 && r2 == q) ? C.z : C.w;
+                    float tq = (h0 && r0 == q) ? t0 : (h1 && r1 == q) ? t1 : (h2 && r2 == q) ? t2 : t3;
+                    st.push(ref, tq);
+                }
+            }
+            cur = (h0 && r0 == 0) ? C.x : (h1 && r1 == 0) ? C.y : (h2 && r2 == 0) ? C.z : C.w;
             return kRayActive;
         }
         if (!in_blas) {

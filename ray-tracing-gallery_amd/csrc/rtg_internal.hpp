@@ -68,9 +68,9 @@ inline void carve_size(size_t& off, size_t count) {
 }
 
 enum StatusBits : uint32_t {
-    kStatusBadIndex = 1u << 0,      // BLAS build: index > max_vertex
-    kStatusBadBlas = 1u << 1,       // TLAS build/refit: reference is not a live BLAS
-    kStatusStackOverflow = 1u << 2, // trace: traversal stack exhausted
+    kStatusBadIndex = RTG_STATUS_BAD_INDEX,            // BLAS build: index > max_vertex
+    kStatusBadBlas = RTG_STATUS_UNKNOWN_BLAS,          // TLAS build/refit: reference is not a live BLAS
+    kStatusStackOverflow = RTG_STATUS_STACK_OVERFLOW,  // trace: traversal stack exhausted
 };
 
 struct TimedLaunch {

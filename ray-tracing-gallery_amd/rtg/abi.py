@@ -42,6 +42,10 @@ RTG_QUERY_OCCLUSION = 1
 
 RTG_TRACE_HEATMAP = 0x1
 
+RTG_STATUS_BAD_INDEX = 0x1
+RTG_STATUS_UNKNOWN_BLAS = 0x2
+RTG_STATUS_STACK_OVERFLOW = 0x4
+
 RTG_KERNEL_NAMES = ["raygen", "trace_closest", "shade", "trace_occlusion", "resolve", "refit", "batch_graph"]
 RTG_KERNEL_RAYGEN = 0
 RTG_KERNEL_TRACE_CLOSEST = 1

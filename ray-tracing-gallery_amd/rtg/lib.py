@@ -17,7 +17,7 @@ EXPORTS = [
     "rtg_blas_query_sizes", "rtg_blas_build", "rtg_blas_destroy",
     "rtg_tlas_query_sizes", "rtg_tlas_build", "rtg_tlas_refit", "rtg_tlas_clone", "rtg_tlas_destroy",
     "rtg_as_get_info", "rtg_trace_rays", "rtg_trace_image", "rtg_trace_batch",
-    "rtg_set_counters", "rtg_get_counters", "rtg_set_timing", "rtg_get_kernel_time",
+    "rtg_set_counters", "rtg_get_counters", "rtg_get_status", "rtg_set_timing", "rtg_get_kernel_time",
     "rtg_server_create", "rtg_server_add_scene", "rtg_server_submit", "rtg_server_poll", "rtg_server_replay",
     "rtg_server_clock", "rtg_server_get_stats", "rtg_server_destroy",
     "rtg_get_tuning", "rtg_set_tuning", "rtg_autotune",
@@ -59,6 +59,7 @@ def load():
     L.rtg_trace_batch.argtypes = [vp, vp, u32, vp]
     L.rtg_set_counters.argtypes = [vp, i32]
     L.rtg_get_counters.argtypes = [vp, C.POINTER(abi.TraceCounters)]
+    L.rtg_get_status.argtypes = [vp, C.POINTER(u32), i32]
     L.rtg_set_timing.argtypes = [vp, i32]
     L.rtg_get_kernel_time.argtypes = [vp, u32, C.POINTER(C.c_double), C.POINTER(u64)]
     L.rtg_server_create.argtypes = [vp, C.POINTER(abi.ServerConfig), C.POINTER(vp)]
@@ -118,6 +119,12 @@ class Context:
         c = abi.TraceCounters()
         check(load().rtg_get_counters(self.ptr, C.byref(c)), "rtg_get_counters")
         return c
+
+    def status(self, clear=True):
+        """Sticky device status bits (abi.RTG_STATUS_*); synchronises."""
+        v = C.c_uint32()
+        check(load().rtg_get_status(self.ptr, C.byref(v), 1 if clear else 0), "rtg_get_status")
+        return int(v.value)
 
     def set_timing(self, on):
         check(load().rtg_set_timing(self.ptr, 1 if on else 0), "rtg_set_timing")

@@ -33,3 +33,13 @@ def ctx():
     c = Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture(autouse=True)
+def no_stack_overflow(request):
+    """Every GPU test ends with no traversal-stack overflow on its context."""
+    yield
+    if "ctx" in request.fixturenames and request.node.get_closest_marker("gpu"):
+        from rtg import abi
+        c = request.getfixturevalue("ctx")
+        assert not (c.status() & abi.RTG_STATUS_STACK_OVERFLOW)

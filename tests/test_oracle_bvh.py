@@ -1,18 +1,21 @@
-"""Pins the oracle's LBVH build and TLAS refit with structural invariants and a
-brute-force construction of the binary radix tree (Karras 2012) from its
-definition: over keys sorted by (Morton code, primitive index), the node
+"""Pins the oracle's BVH build and TLAS refit with structural invariants and a
+brute-force construction from the definitions: the binary radix tree of
+Karras 2012 (over keys sorted by (Morton code, primitive index), the node
 covering leaves [i, j] splits where the common prefix of the 62-bit augmented
-keys of i and j ends."""
+keys of i and j ends), collapsed top-down into 4-wide nodes by the
+largest-surface-area rule, breadth first, with subtrees of at most
+BLAS_LEAF_MAX primitives as leaves."""
 import ctypes as C
 
 import numpy as np
 import pytest
 
 import oracle_host as oh
-from blob import EMPTY, FIRST, LEAF, Blob
+from blob import EMPTY, FIRST, LEAF, Blob, leaf_span
 from rtg import scenes
 
 L = oh.lib()
+BLAS_LEAF_MAX = 1  # primitives per BLAS leaf at most
 
 
 def tri_boxes(mesh):
@@ -77,8 +80,58 @@ def brute_radix_tree(aug):
     return splits
 
 
+def f32_area(lo, hi):
+    e = [np.float32(hi[a]) - np.float32(lo[a]) for a in range(3)]
+    return np.float32(np.float32(e[0] * e[1]) + np.float32(e[1] * e[2])) + np.float32(e[2] * e[0])
+
+
+def collapse_reference(splits, leaf_boxes, leaf_max):
+    """Child refs of the 4-wide nodes, breadth first, from the binary tree's
+    splits: a node rooted at binary range r takes r's two halves and, while
+    it has fewer than four slots, opens the slot of largest box area (first
+    slot on ties) among subtrees of more than leaf_max leaves."""
+    n = len(leaf_boxes)
+    if n <= max(1, leaf_max):
+        return [[LEAF | ((n - 1) << 27) if n else EMPTY, EMPTY, EMPTY, EMPTY]]
+
+    def halves(r):
+        i, j = r
+        s = splits[(i, j)]
+        return [(i, s), (s + 1, j)]
+
+    def area(r):
+        i, j = r
+        return f32_area(leaf_boxes[i:j + 1, :3].min(0), leaf_boxes[i:j + 1, 3:].max(0))
+
+    def openable(r):
+        return r[1] - r[0] + 1 > leaf_max
+
+    out, queue, head = [], [(0, n - 1)], 0
+    while head < len(queue):
+        r = queue[head]
+        head += 1
+        slots = halves(r)
+        while len(slots) < 4:
+            best = None
+            for k, c in enumerate(slots):
+                if openable(c) and (best is None or area(c) > area(slots[best])):
+                    best = k
+            if best is None:
+                break
+            slots[best:best + 1] = halves(slots[best])
+        refs = []
+        for c in slots:
+            if openable(c):
+                refs.append(len(queue))
+                queue.append(c)
+            else:
+                refs.append(LEAF | ((c[1] - c[0]) << 27) | c[0])
+        out.append(refs + [EMPTY] * (4 - len(refs)))
+    return out
+
+
 @pytest.mark.parametrize("kind", ["torus", "soup", "dups"])
-def test_blas_is_the_radix_tree_of_sorted_morton_keys(kind):
+def test_blas_is_the_collapsed_radix_tree_of_sorted_morton_keys(kind):
     if kind == "torus":
         mesh = scenes.torus(24, 12)
     elif kind == "soup":
@@ -98,29 +151,37 @@ def test_blas_is_the_radix_tree_of_sorted_morton_keys(kind):
     # the tree's tie-break for equal codes is the position in the sorted array
     aug = [(int(codes[p]) << 32) | k for k, p in enumerate(order)]
     splits = brute_radix_tree(aug)
-    ranges = blob.leaf_ranges()
-    assert len(ranges) == n - 1
-    for node, (first, last, left_end) in ranges.items():
-        assert splits[(first, last)] == left_end, node
+    expect = collapse_reference(splits, boxes[order].astype(np.float32), BLAS_LEAF_MAX)
+    assert int(blob.h["node_count"]) == len(expect)
+    got = [[int(c) for c in nd["child"]] for nd in blob.nodes]
+    assert got == expect
+    assert int(blob.h["node_width"]) == 4
+    assert int(blob.h["max_depth"]) == max(d for _, _, _, d in blob.walk())  # walk() numbers the root level 1
 
 
 def test_blas_boxes_are_exact_unions():
     mesh = scenes.icosphere(2)
     blob = Blob(oh.build_blas(mesh).bytes())
     boxes = tri_boxes(mesh)
+    covered = []
     for node, s, ref, _ in blob.walk():
         lo, hi = blob.child_box(node, s)
-        if ref & LEAF:
-            t = blob.prims[ref & FIRST]
-            v = np.stack([t["v0"], t["v1"], t["v2"]])
+        if ref == EMPTY:
+            assert np.all(lo == np.inf) and np.all(hi == -np.inf)
+        elif ref & LEAF:
+            f, c = leaf_span(ref)
+            assert 1 <= c <= BLAS_LEAF_MAX
+            covered.extend(range(f, f + c))
+            t = blob.prims[f:f + c]
+            v = np.concatenate([t["v0"], t["v1"], t["v2"]])
             assert np.array_equal(lo, v.min(0)) and np.array_equal(hi, v.max(0))
         else:
-            a = blob.child_box(ref, 0)
-            b = blob.child_box(ref, 1)
-            assert np.array_equal(lo, np.minimum(a[0], b[0])) and np.array_equal(hi, np.maximum(a[1], b[1]))
+            kids = [blob.child_box(ref, k) for k in range(4)]
+            assert np.array_equal(lo, np.min([k[0] for k in kids], 0))
+            assert np.array_equal(hi, np.max([k[1] for k in kids], 0))
+    assert sorted(covered) == list(range(len(boxes)))  # every leaf position exactly once
     assert np.array_equal(blob.h["lo"], boxes[:, :3].min(0))
     assert np.array_equal(blob.h["hi"], boxes[:, 3:].max(0))
-    assert int(blob.h["max_depth"]) == max(d for _, _, _, d in blob.walk())
 
 
 def test_blas_geometry_ids_and_opacity():
@@ -140,15 +201,26 @@ def test_blas_geometry_ids_and_opacity():
     assert len(seen) == len(ix)
 
 
-@pytest.mark.parametrize("n", [0, 1])
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 5, 17])
 def test_tiny_blas(n):
+    """n copies of one triangle: equal Morton codes, so the radix tree splits
+    the sorted positions at their highest differing bit."""
     mesh = scenes.Mesh("tiny", [[0, 0, 0], [1, 0, 0], [0, 1, 0]], [[0, 0, 1]] * 3, [[0, 0]] * 3,
                        [(np.array([[0, 1, 2]] * n).reshape(-1, 3), True)])
     blob = Blob(oh.build_blas(mesh).bytes())
-    nd = blob.nodes[0]
-    assert int(nd["child"][1]) == EMPTY
-    assert int(nd["child"][0]) == (LEAF if n == 1 else EMPTY)
-    assert int(blob.h["node_count"]) == 1
+    assert [int(t["prim"]) for t in blob.prims] == list(range(n))  # equal keys keep the input order
+    if n <= BLAS_LEAF_MAX or n <= 1:
+        # one node, one leaf of all primitives (or none)
+        assert [int(c) for c in blob.nodes[0]["child"]] == [LEAF | ((n - 1) << 27) if n else EMPTY, EMPTY, EMPTY,
+                                                             EMPTY]
+        assert int(blob.h["node_count"]) == 1
+        return
+    splits = brute_radix_tree(list(range(n)))
+    expect = collapse_reference(splits, tri_boxes(mesh).astype(np.float32), BLAS_LEAF_MAX)
+    assert [[int(c) for c in nd["child"]] for nd in blob.nodes] == expect
+    if n == 5:
+        # 0-3 | 4, then the first of the equal-area halves is opened first
+        assert expect[0] == [LEAF | 0, LEAF | 1, 1, LEAF | 4]
 
 
 def test_bad_index_is_rejected():
@@ -188,11 +260,19 @@ def test_tlas_refit_equals_fresh_build_bounds():
     # every node box is the union of its children, leaves are their instance boxes
     for node, slot, ref, _ in refit.walk():
         lo, hi = refit.child_box(node, slot)
+        if ref == EMPTY:
+            continue
         if ref & LEAF:
+            assert leaf_span(ref)[1] == 1
             assert np.array_equal(np.concatenate([lo, hi]), refit.leafbox[ref & FIRST])
         else:
-            a, b = refit.child_box(ref, 0), refit.child_box(ref, 1)
-            assert np.array_equal(lo, np.minimum(a[0], b[0])) and np.array_equal(hi, np.maximum(a[1], b[1]))
+            kids = [refit.child_box(ref, k) for k in range(4)]
+            assert np.array_equal(lo, np.min([k[0] for k in kids], 0))
+            assert np.array_equal(hi, np.max([k[1] for k in kids], 0))
+    # the refit keeps the topology and the level data of the build
+    assert [list(nd["child"]) for nd in refit.nodes] == [list(nd["child"]) for nd in Blob(
+        oh.HostScene(s).tlas.bytes()).nodes]
+    assert refit.level_off[0] == 0 and refit.level_off[-1] == int(refit.h["node_count"])
     # containment: every instance's transformed BLAS box lies in its leaf box
     blas_info = [b.info() for b in hs.blas]
     for k, r in enumerate(refit.prims):
