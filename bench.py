@@ -37,6 +37,18 @@ def algorithmic_bytes(c, kind):
     return rays * (32 + 4) + c["occ_node_visits"] * 128 + c["occ_triangle_tests"] * 48 + c["occ_instance_visits"] * 112
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by
+    the copy-kernel calibration) from the committed rocprofv3 summary of this
+    benchmark, or None."""
+    path = os.path.join(REPO, "profiles", "hbm_traffic_trace_kernels.json")
+    try:
+        v = json.load(open(path))["per_launch_bytes"][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return int(v["fetch"] + v["write"])
+
+
 def cpu_baseline(scene, threads):
     """The CPU oracle tracing frame 0 of the same scene on the host cores."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -268,9 +280,11 @@ def incoherent_benchmark(ctx, n_rays=16 << 20, sample=65536):
         else:
             nb = n_rays * (32 + 4) + c.occ_node_visits * 128 + c.occ_triangle_tests * 48 + c.occ_instance_visits * 112
             nodes, trisv = c.occ_node_visits, c.occ_triangle_tests
+        traffic = measured_traffic(f"k_trace_rays_{name}")
         out[name] = {"ms": round(ms, 3), "mrays_s": round(n_rays / (ms / 1e3) / 1e6, 1),
                      "nodes_per_ray": round(nodes / n_rays, 2), "tris_per_ray": round(trisv / n_rays, 2),
-                     "algorithmic_gb_s": round(nb / (ms / 1e3) / 1e9, 1)}
+                     "algorithmic_gb_s": round(nb / (ms / 1e3) / 1e9, 1),
+                     "measured_traffic_gb_s": round(traffic / (ms / 1e3) / 1e9, 1) if traffic else None}
     torch.cuda.synchronize()
     # bit-exact check of a ray sample against the oracle (same BVH, same order)
     sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -383,7 +397,7 @@ def main():
     achieved = bytes_total / (ms / 1e3) / 1e9 if ms > 0 else 0.0
     per_launch_bytes = bytes_total / max(1, launches)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(f"k_trace_{dom}"),
                 "kernel": f"k_trace_{'closest' if dom == 'closest' else 'occlusion'}",
                 "avg_launch_ms": round(ms / max(1, launches), 4),
                 "algorithmic_bytes_per_launch": int(per_launch_bytes)}
