@@ -304,6 +304,11 @@ typedef struct rtg_trace_opts {
 /* Heatmap mode: colour = heatmap_temperature(cycles / 1e6) (lib.rs:174-186);
  * also enabled by uniforms.show_heatmap. */
 #define RTG_TRACE_HEATMAP 0x1u
+/* out_rgba8 in B8G8R8A8 byte order instead of R8G8B8A8: the storage image
+ * takes the surface format, B8G8R8A8_UNORM when the surface offers it
+ * (src/util_functions.rs:67-73, src/main.rs:606-611), and is copied to the
+ * swapchain image as is (src/command_buffer_recording.rs:166-179). */
+#define RTG_TRACE_OUTPUT_BGRA8 0x2u
 
 typedef struct rtg_image_job {
     rtg_tlas tlas;
@@ -365,6 +370,26 @@ int rtg_get_status(rtg_ctx* ctx, uint32_t* bits, int clear);
 #define RTG_KERNEL_COUNT 7u
 int rtg_set_timing(rtg_ctx* ctx, int enable);
 int rtg_get_kernel_time(rtg_ctx* ctx, uint32_t kind, double* total_ms, uint64_t* launches);
+
+/* ------------------------------------------------------------------------ */
+/* Frame driver: frames in flight over one scene.                           */
+/* ≙ the reference's two Frame resources, each with a TLAS clone, a mapped  */
+/* instance buffer and an output image (src/main.rs:600-668), and its frame */
+/* loop: wait for the frame's fence, write the instance transforms, refit   */
+/* that frame's TLAS, trace (src/main.rs:916-996, src/scene.rs:167-204).    */
+/* ------------------------------------------------------------------------ */
+typedef struct rtg_frames rtg_frames;
+/* frames_in_flight (1..8) slots, each a copy of `tlas` with its own pinned
+ * host instance array (seeded with the TLAS's instances) and frame state. */
+int rtg_frames_create(rtg_ctx* ctx, rtg_tlas tlas, uint32_t frames_in_flight, rtg_frames** out);
+/* Waits until the slot of `frame` (frame % frames_in_flight) has finished its
+ * last frame and returns its host instance array (write this frame's
+ * transforms there) and its TLAS. */
+int rtg_frames_begin(rtg_frames* frames, uint64_t frame, rtg_instance** host_instances, rtg_tlas* slot_tlas);
+/* On `stream`: uploads the slot's instances, refits its TLAS and traces `job`
+ * with the slot's TLAS in place of job->tlas. */
+int rtg_frames_submit(rtg_frames* frames, uint64_t frame, const rtg_image_job* job, void* stream);
+int rtg_frames_destroy(rtg_frames* frames);
 
 /* ------------------------------------------------------------------------ */
 /* Launch tuning and the autotuner.                                         */

@@ -944,6 +944,11 @@ static void image_item(void* a, qctx* q, uint32_t item) {
             uint32_t r = (uint32_t)(clampf(sr[0], 0.0f, 1.0f) * 255.0f + 0.5f);
             uint32_t g = (uint32_t)(clampf(sr[1], 0.0f, 1.0f) * 255.0f + 0.5f);
             uint32_t b = (uint32_t)(clampf(sr[2], 0.0f, 1.0f) * 255.0f + 0.5f);
+            if (job->opts.flags & RTG_TRACE_OUTPUT_BGRA8) {
+                uint32_t t = r;
+                r = b;
+                b = t;
+            }
             job->out_rgba8[pix] = r | (g << 8) | (b << 16) | (255u << 24);
         }
     }

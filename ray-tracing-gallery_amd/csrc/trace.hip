@@ -758,6 +758,11 @@ __global__ __launch_bounds__(256) void k_resolve(Frame F, uint64_t total_pixels)
             uint32_t r = (uint32_t)(clampf(srgb1(col.x), 0.0f, 1.0f) * 255.0f + 0.5f);
             uint32_t g = (uint32_t)(clampf(srgb1(col.y), 0.0f, 1.0f) * 255.0f + 0.5f);
             uint32_t b = (uint32_t)(clampf(srgb1(col.z), 0.0f, 1.0f) * 255.0f + 0.5f);
+            if (J.flags & RTG_TRACE_OUTPUT_BGRA8) {
+                uint32_t t = r;
+                r = b;
+                b = t;
+            }
             J.out_rgba8[pix] = r | (g << 8) | (b << 16) | (255u << 24);
         }
     }
@@ -915,6 +920,8 @@ static int validate_job(rtg_ctx* ctx, const rtg_image_job& j, uint32_t k, bool c
                 "job %u: max_traces %u (1..4)", k, j.opts.max_traces);
     RTG_REQUIRE(j.opts.shadow_rays >= 1 && j.opts.shadow_rays <= 16, RTG_ERROR_INVALID_ARGUMENT,
                 "job %u: shadow_rays %u (1..16)", k, j.opts.shadow_rays);
+    RTG_REQUIRE((j.opts.flags & ~(RTG_TRACE_HEATMAP | RTG_TRACE_OUTPUT_BGRA8)) == 0, RTG_ERROR_INVALID_ARGUMENT,
+                "job %u: unknown trace flags 0x%x", k, j.opts.flags);
     RTG_REQUIRE(j.scene.model_count == 0 || j.scene.model_info, RTG_ERROR_INVALID_ARGUMENT,
                 "job %u: model_count %u with a null model table", k, j.scene.model_count);
     RTG_REQUIRE(j.scene.texture_count == 0 || j.scene.textures, RTG_ERROR_INVALID_ARGUMENT,

@@ -41,6 +41,7 @@ RTG_QUERY_CLOSEST = 0
 RTG_QUERY_OCCLUSION = 1
 
 RTG_TRACE_HEATMAP = 0x1
+RTG_TRACE_OUTPUT_BGRA8 = 0x2
 
 RTG_STATUS_BAD_INDEX = 0x1
 RTG_STATUS_UNKNOWN_BLAS = 0x2

@@ -22,6 +22,7 @@ EXPORTS = [
     "rtg_server_clock", "rtg_server_get_stats", "rtg_server_destroy",
     "rtg_get_tuning", "rtg_set_tuning", "rtg_autotune",
     "rtg_batch_create", "rtg_batch_update", "rtg_batch_launch", "rtg_batch_destroy",
+    "rtg_frames_create", "rtg_frames_begin", "rtg_frames_submit", "rtg_frames_destroy",
 ]
 
 _lib = None
@@ -78,6 +79,10 @@ def load():
     L.rtg_batch_update.argtypes = [vp, vp, u32, vp]
     L.rtg_batch_launch.argtypes = [vp, vp]
     L.rtg_batch_destroy.argtypes = [vp]
+    L.rtg_frames_create.argtypes = [vp, u64, u32, C.POINTER(vp)]
+    L.rtg_frames_begin.argtypes = [vp, u64, C.POINTER(C.POINTER(abi.Instance)), C.POINTER(u64)]
+    L.rtg_frames_submit.argtypes = [vp, u64, C.POINTER(abi.ImageJob), vp]
+    L.rtg_frames_destroy.argtypes = [vp]
     for name in EXPORTS:
         if name not in ("rtg_destroy", "rtg_last_error", "rtg_abi_version"):
             getattr(L, name).restype = C.c_int

@@ -204,3 +204,28 @@ def test_incoherent_full_scene(ctx):
         for f in names:
             assert getattr(c, f) == getattr(oc, f), f
     ds.destroy()
+
+
+def test_bgra_output_and_png(ctx, small, tmp_path):
+    """The output step: BGRA8 byte order as the reference's swapchain format,
+    equal to the oracle's, and a PNG of the frame that decodes to it."""
+    from rtg import image
+    s, ds, hs = small
+    o = s.trace_opts()
+    o.flags = abi.RTG_TRACE_OUTPUT_BGRA8
+    outs = ds.trace_image(opts=o)
+    torch.cuda.synchronize()
+    o_rgba, _, _, _ = hs.trace_image(opts=o)
+    assert np.array_equal(outs[0].cpu().numpy().view(np.uint32), o_rgba)
+    img = image.readback(outs[0], s.width, s.height, o.flags)
+    p = tmp_path / "frame.png"
+    image.write_png(str(p), img)
+    assert np.array_equal(image.read_png(str(p)), image.unpack(o_rgba, s.width, s.height, bgra=True))
+
+
+def test_unknown_trace_flags_are_refused(ctx, small):
+    s, ds, hs = small
+    o = s.trace_opts()
+    o.flags = 0x100
+    with pytest.raises(Exception):
+        ds.trace_image(opts=o)
