@@ -84,14 +84,16 @@ struct FrameArena;  // trace.hip
 // the environment: RTG_REFILL_CLOSEST, RTG_REFILL_OCCLUSION (0 selects the
 // non-persistent kernels), RTG_PATH_ORDER, RTG_TRACE_BLOCKS_PER_CU,
 // RTG_FEED_CHUNK).
-// Measured on MI355X (1080p large scene): refilling lanes before the whole
-// wave is idle mixes incoherent rays and loses more than it gains, so the
-// default refills only an idle wave, one 64-ray batch per atomic.
+// Measured on MI355X (1080p large scene, 4-wide BVH): refilling lanes before
+// the whole wave is idle loses 13-16 % on the occlusion kernel, so the
+// default refills only an idle wave, one 64-ray batch per atomic; four
+// persistent workgroups per CU match the traversal kernels' occupancy
+// (4 waves per SIMD at their register count) and beat 5 and 8 slightly.
 struct Tuning {
     int refill_closest = 64;
     int refill_occlusion = 64;
     int path_order = 1;
-    int blocks_per_cu = 5;
+    int blocks_per_cu = 4;
     int feed_chunk = 64;
 };
 

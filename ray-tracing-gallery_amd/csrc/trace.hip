@@ -827,7 +827,7 @@ size_t frame_bytes(uint64_t paths, uint64_t shadow, uint32_t njobs) {
 
 static int trace_grid(rtg_ctx* ctx, const Tuning& tune) {
     // persistent grid: blocks_per_cu blocks of 256 lanes per CU (the LDS
-    // stack takes 32 KB per block, so at most 5 are resident)
+    // stack takes 32 KB per block, so at most 5 fit; registers allow 4)
     return ctx->cu_count * tune.blocks_per_cu;
 }
 static int trace_grid(rtg_ctx* ctx) { return trace_grid(ctx, ctx->tune); }

@@ -273,7 +273,7 @@ class Tuning(C.Structure):
         return {name: int(getattr(self, name)) for name, _ in self._fields_ if name != "reserved"}
 
     @staticmethod
-    def make(blocks_per_cu=5, feed_chunk=64, refill_closest=64, refill_occlusion=64, path_order=1):
+    def make(blocks_per_cu=4, feed_chunk=64, refill_closest=64, refill_occlusion=64, path_order=1):
         return Tuning(blocks_per_cu, feed_chunk, refill_closest, refill_occlusion, path_order)
 
 

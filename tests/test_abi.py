@@ -88,5 +88,5 @@ def test_tuning_layout():
     assert C.sizeof(abi.Tuning) == 32
     assert abi.Tuning.path_order.offset == 16
     t = abi.Tuning.make()
-    assert t.as_dict() == {"blocks_per_cu": 5, "feed_chunk": 64, "refill_closest": 64, "refill_occlusion": 64,
+    assert t.as_dict() == {"blocks_per_cu": 4, "feed_chunk": 64, "refill_closest": 64, "refill_occlusion": 64,
                            "path_order": 1}

@@ -229,3 +229,24 @@ def test_unknown_trace_flags_are_refused(ctx, small):
     o.flags = 0x100
     with pytest.raises(Exception):
         ds.trace_image(opts=o)
+
+
+def test_trace_image_denser_scene_frames(ctx):
+    """A denser reduced animated scene (200 objects, 16 alpha fences, 8
+    mirrors, 4 shadow rays): frame 2 and a two-sample frame 5 equal the
+    oracle's."""
+    s = scenes.large_scene(width=160, height=90, n_objects=200, n_fences=16, n_mirrors=8)
+    ds = DeviceScene(ctx, s)
+    hs = oh.HostScene(s)
+    s.set_frame(2)
+    ds.refit(ds.instance_buffer(s.transforms_for_frame(2)))
+    _compare_image(ds, hs_refit(hs, s), frame=2)
+    o = s.trace_opts()
+    o.spp = 2
+    _compare_image(ds, hs, frame=5, opts=o)
+    ds.destroy()
+
+
+def hs_refit(hs, s):
+    hs.refit([i.transform for i in s.instances])
+    return hs
