@@ -61,12 +61,20 @@ def cpu_baseline(scene, threads):
     return hs, (rgba, rgb, hits), rays / dt / 1e6, dt
 
 
-def serving_benchmark(ctx, n_scenes=32, n_requests=160, seed=0x5EED0004):
+# (requests per batch, batches in flight, samples per batch): plain = one
+# request per launch, one at a time; served = continuous batching with up to
+# 3 batches in flight of at most 2 M samples each (the setting with the best
+# p99 in a sweep on MI355X; larger batches delay small requests)
+SERVE_MODES = {"plain": (1, 1, 0), "served": (32, 3, 2 << 20)}
+
+
+def serving_benchmark(ctx, n_scenes=32, n_requests=160, seed=0x5EED0004, modes=None):
     """Served stream: n_scenes small-tori scenes resident in HBM; requests of
     256x256, 1280x720 or 1920x1080 with 1-8 samples per pixel, each with a
     random instance-transform update (a refit of a private TLAS copy).
     Plain = one request per launch, one at a time; served = continuous
-    batching (up to 32 requests per batch, 2 batches in flight). Returns
+    batching (up to 32 requests and 2 M samples per batch, 3 batches in
+    flight). Returns
     saturation throughput (all requests at t=0) and p50/p99 latency at a
     Poisson arrival rate of 0.6x the plain path's saturation throughput."""
     import numpy as np
@@ -101,35 +109,35 @@ def serving_benchmark(ctx, n_scenes=32, n_requests=160, seed=0x5EED0004):
     log(f"serving setup {time.perf_counter() - t0:.1f}s: {n_scenes} scenes, {n_requests} requests, "
         f"{paths / 1e6:.1f} M samples")
 
-    def run(max_batch, max_inflight, arrivals):
-        srv = Server(ctx, budget_bytes=48 << 30, max_batch=max_batch, max_inflight=max_inflight)
-        for ds in resident:
-            srv.add_scene(ds)
+    def replay(srv, arrivals):
         comp = srv.replay(reqs[:len(arrivals)], arrivals)
-        stats = srv.stats()
-        srv.close()
         assert all(c.status == 0 for c in comp)
         lat = np.array([c.done_s - c.submit_s for c in comp]) * 1e3
         span = max(c.done_s for c in comp) - min(c.submit_s for c in comp)
-        return lat, span, stats
+        return lat, span, max(c.batch_size for c in comp), len({c.start_s for c in comp})
 
-    modes = {"plain": (1, 1), "served": (32, 2)}
-    # warm both paths (arenas, code objects)
-    for mb, mi in modes.values():
-        run(mb, mi, np.zeros(min(16, n_requests)))
-    sat = {}
-    for name, (mb, mi) in modes.items():
-        lat, span, stats = run(mb, mi, np.zeros(n_requests))
+    # each mode: one server, warmed (arenas sized, code loaded), then all
+    # requests at t=0 (saturation), then the same requests as Poisson
+    # arrivals at 0.6x the plain path's saturation throughput
+    modes = dict(modes or SERVE_MODES)
+    assert next(iter(modes)) == "plain"
+    sat, poisson, rate, arrivals = {}, {}, None, None
+    for name, (mb, mi, ms) in modes.items():
+        srv = Server(ctx, budget_bytes=48 << 30, max_batch=mb, max_inflight=mi, max_samples=ms)
+        for ds in resident:
+            srv.add_scene(ds)
+        replay(srv, np.zeros(min(16, n_requests)))
+        lat, span, maxb, nb = replay(srv, np.zeros(n_requests))
         sat[name] = {"throughput_rps": round(n_requests / span, 2), "makespan_s": round(span, 3),
-                     "max_batch": stats["max_batch"], "batches": stats["batches"]}
-    rate = 0.6 * sat["plain"]["throughput_rps"]
-    arrivals = np.cumsum(-np.log(1.0 - scenes.Rng(seed + 99).uniform(n_requests)) / rate)
-    poisson = {}
-    for name, (mb, mi) in modes.items():
-        lat, span, stats = run(mb, mi, arrivals)
+                     "max_batch": maxb, "batches": nb}
+        if rate is None:
+            rate = 0.6 * sat["plain"]["throughput_rps"]
+            arrivals = np.cumsum(-np.log(1.0 - scenes.Rng(seed + 99).uniform(n_requests)) / rate)
+        lat, span, maxb, nb = replay(srv, arrivals)
         poisson[name] = {"p50_ms": round(float(np.percentile(lat, 50)), 2),
                          "p99_ms": round(float(np.percentile(lat, 99)), 2),
-                         "throughput_rps": round(n_requests / span, 2), "max_batch": stats["max_batch"]}
+                         "throughput_rps": round(n_requests / span, 2), "max_batch": maxb}
+        srv.close()
     # exact ray count of the request set (counters of a plain replay)
     ctx.set_counters(True)
     rays = 0
@@ -150,7 +158,8 @@ def serving_benchmark(ctx, n_scenes=32, n_requests=160, seed=0x5EED0004):
     return {"scenes": n_scenes, "requests": n_requests, "samples": paths, "rays": int(rays),
             "resolutions": ["256x256", "1280x720", "1920x1080"], "spp": "1-8",
             "saturation": sat, "poisson_rate_rps": round(rate, 2), "poisson": poisson,
-            "served_vs_plain_throughput": round(sat["served"]["throughput_rps"] / sat["plain"]["throughput_rps"], 3)}
+            "served_vs_plain_throughput": (round(sat["served"]["throughput_rps"] / sat["plain"]["throughput_rps"], 3)
+                                           if "served" in sat and "plain" in sat else None)}
 
 
 def batched_benchmark(ctx, steps, seed=0x5EED0003):
@@ -336,6 +345,8 @@ def main():
     from rtg import abi, scenes
     from rtg.device import DeviceScene, hits_to_numpy
     from rtg.lib import Context
+
+    torch.cuda.init()  # torch's device state first, then the library's context
 
     if args.copy_calibration:
         n = args.copy_calibration // 4

@@ -448,9 +448,12 @@ int rtg_batch_destroy(rtg_batch* batch);
 typedef struct rtg_server rtg_server;
 
 typedef struct rtg_server_config {
-    uint64_t hbm_budget_bytes; /* frame state + TLAS copies of all in-flight batches */
-    uint32_t max_batch_jobs;   /* requests per batch; 1 = one request per launch */
-    uint32_t max_inflight;     /* batches in flight at once (1..8) */
+    uint64_t hbm_budget_bytes;  /* frame state + TLAS copies of all in-flight batches */
+    uint32_t max_batch_jobs;    /* requests per batch; 1 = one request per launch */
+    uint32_t max_inflight;      /* batches in flight at once (1..8) */
+    uint64_t max_batch_samples; /* samples (pixels x spp) per batch, the first request excepted; 0 = no cap.
+                                   Every request of a batch completes when the batch does, so a cap bounds
+                                   how long small requests wait behind large ones. */
 } rtg_server_config;
 
 typedef struct rtg_request {
@@ -524,6 +527,7 @@ static_assert(sizeof(rtg_ray) == 32, "ray is 32 bytes");
 static_assert(sizeof(rtg_hit) == 24, "hit is 24 bytes");
 static_assert(sizeof(rtg_texture) == 24, "texture descriptor is 24 bytes");
 static_assert(sizeof(rtg_tuning) == 32, "tuning is 32 bytes");
+static_assert(sizeof(rtg_server_config) == 24, "server config is 24 bytes");
 #else
 _Static_assert(sizeof(rtg_instance) == 64, "instance record is 64 bytes");
 _Static_assert(sizeof(rtg_uniforms) == 176, "Uniforms is 176 bytes");

@@ -7,8 +7,9 @@
 // transforms, gets a private copy of its scene's TLAS that is refitted, and
 // is traced together with the other requests of its batch. A worker thread
 // keeps up to max_inflight batches on the device (one stream and one frame
-// arena per slot) and forms a new batch from the pending requests whenever a
-// slot is free, as long as the batch fits the slot's share of the budget.
+// arena per slot) and forms a new batch from the pending requests, in
+// arrival order, whenever a slot is free, as long as the batch fits the
+// slot's share of the budget and the configured number of samples.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -172,6 +173,7 @@ void rtg_server::run() {
                 size_t nc = copies + (r.instances ? ((p.scene.tlas_bytes + 255) & ~uint64_t(255)) : 0);
                 uint64_t nb = frame_bytes(np, ns, (uint32_t)s.batch.size() + 1) + nc;
                 if (!s.batch.empty() && nb > slot_capacity) break;
+                if (!s.batch.empty() && cfg.max_batch_samples && np > cfg.max_batch_samples) break;
                 paths = np;
                 shadow = ns;
                 copies = nc;

@@ -281,7 +281,11 @@ assert C.sizeof(Tuning) == 32
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_jobs", C.c_uint32), ("max_inflight", C.c_uint32)]
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_jobs", C.c_uint32), ("max_inflight", C.c_uint32),
+                ("max_batch_samples", C.c_uint64)]
+
+
+assert C.sizeof(ServerConfig) == 24
 
 
 class Request(C.Structure):

@@ -14,8 +14,8 @@ from .lib import check, load
 
 
 class Server:
-    def __init__(self, ctx, budget_bytes=8 << 30, max_batch=16, max_inflight=2):
-        cfg = abi.ServerConfig(int(budget_bytes), int(max_batch), int(max_inflight))
+    def __init__(self, ctx, budget_bytes=8 << 30, max_batch=16, max_inflight=2, max_samples=0):
+        cfg = abi.ServerConfig(int(budget_bytes), int(max_batch), int(max_inflight), int(max_samples))
         self.ptr = C.c_void_p()
         check(load().rtg_server_create(ctx.ptr, C.byref(cfg), C.byref(self.ptr)), "rtg_server_create")
         self.scenes = []

@@ -107,3 +107,32 @@ def test_budget_rejects_requests_that_never_fit(ctx, resident):
     done = srv.poll(wait_all=True)
     assert len(done) == 1 and done[0].status == 0
     srv.close()
+
+
+def test_sample_cap_bounds_batches(ctx, resident):
+    reqs, outs, bufs = request_set(resident, n=12, seed=5)
+    cap = 6000
+    srv = Server(ctx, budget_bytes=1 << 30, max_batch=8, max_inflight=2, max_samples=cap)
+    for ds in resident:
+        srv.add_scene(ds)
+    comp = srv.replay(reqs, np.zeros(len(reqs)))
+    srv.close()
+    assert all(c.status == 0 for c in comp)
+    batches = {}
+    for c, r in zip(comp, reqs):
+        batches.setdefault(c.start_s, []).append(r.width * r.height * r.opts.spp)
+    assert len(batches) < len(reqs)  # some batching still happens
+    for samples in batches.values():
+        assert len(samples) == 1 or sum(samples) <= cap
+    capped = snapshot(outs)
+    for o in outs:
+        for t in o:
+            t.zero_()
+    srv = Server(ctx, budget_bytes=1 << 30, max_batch=1, max_inflight=1)
+    for ds in resident:
+        srv.add_scene(ds)
+    srv.replay(reqs, np.arange(len(reqs)) * 0.01)
+    srv.close()
+    for a, b in zip(snapshot(outs), capped):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
