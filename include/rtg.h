@@ -404,7 +404,11 @@ typedef struct rtg_tuning {
                                   0 = non-persistent kernel (grid-stride rounds) */
     uint32_t refill_occlusion; /* the same for occlusion rays */
     uint32_t path_order;       /* 0 = row-major paths, 1 = 8x8 pixel blocks per wavefront */
-    uint32_t reserved[3];      /* zero */
+    uint32_t bounce_blocks;    /* persistent workgroups (0..4096) that trace the bounce levels, and the
+                                  shadow rays they add, on a second stream beside the first level's
+                                  shadow rays (taken from the blocks_per_cu grid, at most half of it);
+                                  0 = every level in sequence on the caller's stream */
+    uint32_t reserved[2];      /* zero */
 } rtg_tuning;
 int rtg_get_tuning(rtg_ctx* ctx, rtg_tuning* out);
 int rtg_set_tuning(rtg_ctx* ctx, const rtg_tuning* tuning);

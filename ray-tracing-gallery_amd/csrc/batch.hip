@@ -31,7 +31,9 @@ static int check_tuning(const rtg_tuning& t) {
     RTG_REQUIRE(t.refill_closest <= 64 && t.refill_occlusion <= 64, RTG_ERROR_INVALID_ARGUMENT,
                 "tuning: refill thresholds %u/%u (0..64)", t.refill_closest, t.refill_occlusion);
     RTG_REQUIRE(t.path_order <= 1, RTG_ERROR_INVALID_ARGUMENT, "tuning: path_order %u (0..1)", t.path_order);
-    RTG_REQUIRE(t.reserved[0] == 0 && t.reserved[1] == 0 && t.reserved[2] == 0, RTG_ERROR_INVALID_ARGUMENT,
+    RTG_REQUIRE(t.bounce_blocks <= 4096, RTG_ERROR_INVALID_ARGUMENT, "tuning: bounce_blocks %u (0..4096)",
+                t.bounce_blocks);
+    RTG_REQUIRE(t.reserved[0] == 0 && t.reserved[1] == 0, RTG_ERROR_INVALID_ARGUMENT,
                 "tuning: reserved fields must be zero");
     return RTG_OK;
 }
@@ -43,6 +45,7 @@ static Tuning from_abi(const rtg_tuning& t) {
     u.refill_closest = (int)t.refill_closest;
     u.refill_occlusion = (int)t.refill_occlusion;
     u.path_order = (int)t.path_order;
+    u.bounce_blocks = (int)t.bounce_blocks;
     return u;
 }
 
@@ -54,13 +57,14 @@ static rtg_tuning to_abi(const Tuning& u) {
     t.refill_closest = (uint32_t)u.refill_closest;
     t.refill_occlusion = (uint32_t)u.refill_occlusion;
     t.path_order = (uint32_t)u.path_order;
+    t.bounce_blocks = (uint32_t)u.bounce_blocks;
     return t;
 }
 
 static bool operator<(const Tuning& a, const Tuning& b) {
-    int x[5] = {a.blocks_per_cu, a.feed_chunk, a.refill_closest, a.refill_occlusion, a.path_order};
-    int y[5] = {b.blocks_per_cu, b.feed_chunk, b.refill_closest, b.refill_occlusion, b.path_order};
-    return std::lexicographical_compare(x, x + 5, y, y + 5);
+    int x[6] = {a.blocks_per_cu, a.feed_chunk, a.refill_closest, a.refill_occlusion, a.path_order, a.bounce_blocks};
+    int y[6] = {b.blocks_per_cu, b.feed_chunk, b.refill_closest, b.refill_occlusion, b.path_order, b.bounce_blocks};
+    return std::lexicographical_compare(x, x + 6, y, y + 6);
 }
 
 // Owns a stream and an event pair for the autotuner's timed runs, which use
@@ -137,12 +141,13 @@ extern "C" int rtg_autotune(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t n,
     RTG_HIP(hipEventCreate(&rig.e1));
     // Coordinate search: each pass tries every value of one parameter with
     // the others at the best setting so far; two passes.
-    const std::vector<int> axes[5] = {
-        {4, 5, 6, 8},       // blocks_per_cu
-        {64, 128, 256},     // feed_chunk
-        {64, 48, 32, 0},    // refill_closest
-        {64, 48, 32, 0},    // refill_occlusion
-        {1, 0},             // path_order
+    const std::vector<int> axes[6] = {
+        {4, 5, 6, 8},         // blocks_per_cu
+        {64, 128, 256},       // feed_chunk
+        {64, 48, 32, 0},      // refill_closest
+        {64, 48, 32, 0},      // refill_occlusion
+        {1, 0},               // path_order
+        {128, 256, 64, 0},    // bounce_blocks
     };
     std::map<Tuning, double> seen;
     Tuning cur = ctx->tune;
@@ -151,11 +156,11 @@ extern "C" int rtg_autotune(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t n,
     if (st != RTG_OK) return st;
     seen[cur] = cur_ms;
     for (int pass = 0; pass < 2; ++pass) {
-        for (int axis = 0; axis < 5; ++axis) {
+        for (int axis = 0; axis < 6; ++axis) {
             for (int value : axes[axis]) {
                 Tuning t = cur;
-                int* field[5] = {&t.blocks_per_cu, &t.feed_chunk, &t.refill_closest, &t.refill_occlusion,
-                                 &t.path_order};
+                int* field[6] = {&t.blocks_per_cu, &t.feed_chunk, &t.refill_closest, &t.refill_occlusion,
+                                 &t.path_order, &t.bounce_blocks};
                 *field[axis] = value;
                 if (seen.count(t)) continue;
                 double ms = 0.0;

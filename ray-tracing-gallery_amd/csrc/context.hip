@@ -115,6 +115,7 @@ extern "C" int rtg_create(rtg_ctx** out, int device) {
     ctx->tune.blocks_per_cu = env_int("RTG_TRACE_BLOCKS_PER_CU", ctx->tune.blocks_per_cu, 1, 8);
     // RTG_FEED_CHUNK counts 64-ray batches taken per atomic
     ctx->tune.feed_chunk = 64 * env_int("RTG_FEED_CHUNK", ctx->tune.feed_chunk / 64, 1, 16);
+    ctx->tune.bounce_blocks = env_int("RTG_BOUNCE_BLOCKS", ctx->tune.bounce_blocks, 0, 4096);
     // IEC 61966-2-1 decode of the 256 sRGB codes, in double, rounded once.
     float lut[256];
     for (int i = 0; i < 256; ++i) {

@@ -83,7 +83,7 @@ struct FrameArena;  // trace.hip
 // Launch parameters of the traversal kernels (defaults, overridable through
 // the environment: RTG_REFILL_CLOSEST, RTG_REFILL_OCCLUSION (0 selects the
 // non-persistent kernels), RTG_PATH_ORDER, RTG_TRACE_BLOCKS_PER_CU,
-// RTG_FEED_CHUNK).
+// RTG_FEED_CHUNK, RTG_BOUNCE_BLOCKS).
 // Measured on MI355X (1080p large scene, 4-wide BVH): refilling lanes before
 // the whole wave is idle loses 13-16 % on the occlusion kernel, so the
 // default refills only an idle wave, one 64-ray batch per atomic; four
@@ -95,6 +95,11 @@ struct Tuning {
     int path_order = 1;
     int blocks_per_cu = 4;
     int feed_chunk = 64;
+    // persistent blocks that trace the bounce levels beside the first
+    // level's shadow rays (0: all levels in sequence on the caller's stream);
+    // 128 of 1,024 measured best on the 1080p scene (64 leaves the bounce
+    // chain on the critical path, 256 and more slow the shadow rays)
+    int bounce_blocks = 128;
 };
 
 }  // namespace rtg

@@ -23,6 +23,9 @@ constexpr int kTraceThreads = 256;
 constexpr int kCountSlots = 16;
 constexpr int kFeedLevel = 8;    // counts[kFeedLevel + level]: feed counter of a trace level
 constexpr int kFeedShadow = 12;  // counts[kFeedShadow]: feed counter of the shadow queue
+constexpr int kShadowFirst = 5;      // counts[5]: shadow rays of trace level 0
+constexpr int kFeedShadowRest = 13;  // counts[13]: feed counter of the shadow rays after them
+constexpr int kPartAll = 0, kPartFirst = 1, kPartRest = 2;  // parts of the shadow queue
 
 // Order of an image's pixels along its paths: bands of 8 rows, each band
 // walked column by column, so that the 64 lanes of a wavefront cover an 8x8
@@ -93,8 +96,9 @@ struct Frame {
     uint4* hit_a;        // t bits, instance, primitive, geometry
     float2* hit_b;       // u, v
     Queue sq;            // shadow rays
-    uint32_t* counts;    // [0..3] queue sizes per trace level, [4] shadow rays,
-                         // [8..11] feed counters per trace level, [12] shadow feed counter
+    uint32_t* counts;    // [0..3] queue sizes per trace level, [4] shadow rays, [5] shadow
+                         // rays of level 0, [8..11] feed counters per trace level,
+                         // [12], [13] shadow feed counters
     uint64_t* counters;  // rtg_trace_counters
     uint2* spill;
     uint32_t order;           // pixel order along paths (0 row-major, 1 8x8 bands)
@@ -222,6 +226,10 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_rays(const uint8_t* tla
 __global__ void k_frame_begin(uint32_t* counts, uint64_t* counters, bool reset_counters) {
     if (threadIdx.x < kCountSlots) counts[threadIdx.x] = 0;
     if (reset_counters && threadIdx.x < kCounterSlots) counters[threadIdx.x] = 0;
+}
+
+__global__ void k_shadow_snapshot(uint32_t* counts) {
+    if (threadIdx.x == 0) counts[kShadowFirst] = counts[4];
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -608,10 +616,22 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
 // Occlusion of the shadow rays (TerminateOnFirstHit | SkipClosestHit, hit
 // group offset 1, miss shader 1: closest_hit_textured.glsl:159-172);
 // shadow_ray_miss (lib.rs:33-36) makes the ray count as visible.
+// Parts of the shadow queue an occlusion launch traces: all of it, or the
+// rays of the first trace level (counts[kShadowFirst] of them, the queue's
+// size when the first shade finished) and the rays appended after them.
+__device__ inline void shadow_part(const Frame& F, int part, uint32_t& lo, uint32_t& hi, uint32_t*& feed) {
+    lo = part == kPartRest ? F.counts[kShadowFirst] : 0u;
+    hi = part == kPartFirst ? F.counts[kShadowFirst] : F.counts[4];
+    feed = &F.counts[part == kPartRest ? kFeedShadowRest : kFeedShadow];
+}
+
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion(Frame F, uint32_t* status) {
+__global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion(Frame F, uint32_t* status, int part) {
     __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
-    const uint32_t n = F.counts[4];
+    uint32_t lo, hi;
+    uint32_t* feed;
+    shadow_part(F, part, lo, hi, feed);
+    const uint32_t n = hi - lo;
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
     Stack st;
     st.lds = s_stack + threadIdx.x;
@@ -629,8 +649,9 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion(Frame F, uint
     while (true) {
         uint64_t idle = __ballot(!active);
         if (want_refill(idle, feeder, F.refill_occlusion)) {
-            uint32_t id = feeder.feed(idle, !active, &F.counts[kFeedShadow], n);
+            uint32_t id = feeder.feed(idle, !active, feed, n);
             if (id != ~0u) {
+                id += lo;
                 path = F.sq.path[id];
                 J = &F.jobs[find_job(F.jobs, F.njobs, path, false)];
                 T = tlas_view(J->tlas);
@@ -657,9 +678,12 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion(Frame F, uint
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion_rounds(Frame F, uint32_t* status) {
+__global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion_rounds(Frame F, uint32_t* status, int part) {
     __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
-    const uint32_t n = F.counts[4];
+    uint32_t lo, hi;
+    uint32_t* feed;
+    shadow_part(F, part, lo, hi, feed);
+    const uint32_t n = hi - lo;
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
     Stack st;
     st.lds = s_stack + threadIdx.x;
@@ -669,8 +693,8 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion_rounds(Frame 
     const uint32_t stride = gridDim.x * kTraceThreads;
     const uint32_t rounds = (n + stride - 1) / stride;
     for (uint32_t r = 0; r < rounds; ++r) {
-        uint32_t i = r * stride + gtid;
-        if (i < n) {
+        uint32_t i = lo + r * stride + gtid;
+        if (i < hi) {
             uint32_t p = F.sq.path[i];
             const JobDev& J = F.jobs[find_job(F.jobs, F.njobs, p, false)];
             const TlasView T = tlas_view(J.tlas);
@@ -780,6 +804,9 @@ struct FrameArena {
     JobDev* h_jobs = nullptr;  // pinned staging
     uint32_t h_jobs_cap = 0;
     hipEvent_t staging_free = nullptr;
+    // the bounce levels run on `side` beside the first level's shadow rays
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     Frame F;
     JobDev* d_jobs = nullptr;
     uint32_t* d_counts = nullptr;
@@ -795,6 +822,12 @@ void delete_frame_arena(FrameArena* a) {
     if (a->staging_free) hipEventSynchronize(a->staging_free);
     if (a->h_jobs) hipHostFree(a->h_jobs);
     if (a->staging_free) hipEventDestroy(a->staging_free);
+    if (a->side) {
+        hipStreamSynchronize(a->side);
+        hipStreamDestroy(a->side);
+    }
+    if (a->fork) hipEventDestroy(a->fork);
+    if (a->join) hipEventDestroy(a->join);
     delete a;
 }
 
@@ -903,6 +936,9 @@ static int ensure_frame(rtg_ctx* ctx, FrameArena* a, uint64_t paths, uint64_t sh
         a->h_jobs_cap = njobs;
     }
     if (!a->staging_free) RTG_HIP(hipEventCreateWithFlags(&a->staging_free, hipEventDisableTiming));
+    if (!a->side) RTG_HIP(hipStreamCreateWithFlags(&a->side, hipStreamNonBlocking));
+    if (!a->fork) RTG_HIP(hipEventCreateWithFlags(&a->fork, hipEventDisableTiming));
+    if (!a->join) RTG_HIP(hipEventCreateWithFlags(&a->join, hipEventDisableTiming));
     return RTG_OK;
 }
 
@@ -1012,47 +1048,75 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
         LaunchTimer t(ctx, RTG_KERNEL_RAYGEN, s, timed);
         hipLaunchKernelGGL(k_raygen, dim3(Gsmall), dim3(256), 0, s, F, cnt);
     }
-    for (uint32_t level = 0; level < P.max_traces; ++level) {
+    // trace level `level` (closest hits, then the hit and miss stages) on
+    // `st` with `g` persistent blocks
+    auto level_step = [&](const Frame& Fl, uint32_t level, unsigned g, unsigned gshade, hipStream_t st) {
         const uint32_t* cp = level == 0 ? nullptr : a->d_counts + level;
         uint32_t fixed = level == 0 ? (uint32_t)P.paths : 0u;
         {
-            LaunchTimer t(ctx, RTG_KERNEL_TRACE_CLOSEST, s, timed);
+            LaunchTimer t(ctx, RTG_KERNEL_TRACE_CLOSEST, st, timed);
             if (tune.refill_closest == 0) {
                 if (cnt)
-                    hipLaunchKernelGGL(k_trace_closest_rounds<true>, dim3(G), dim3(kTraceThreads), 0, s, F,
+                    hipLaunchKernelGGL(k_trace_closest_rounds<true>, dim3(g), dim3(kTraceThreads), 0, st, Fl,
                                        (int)level, cp, fixed, ctx->d_status);
                 else
-                    hipLaunchKernelGGL(k_trace_closest_rounds<false>, dim3(G), dim3(kTraceThreads), 0, s, F,
+                    hipLaunchKernelGGL(k_trace_closest_rounds<false>, dim3(g), dim3(kTraceThreads), 0, st, Fl,
                                        (int)level, cp, fixed, ctx->d_status);
             } else if (cnt) {
-                hipLaunchKernelGGL(k_trace_closest<true>, dim3(G), dim3(kTraceThreads), 0, s, F, (int)level, cp, fixed,
-                                   ctx->d_status);
+                hipLaunchKernelGGL(k_trace_closest<true>, dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp,
+                                   fixed, ctx->d_status);
             } else {
-                hipLaunchKernelGGL(k_trace_closest<false>, dim3(G), dim3(kTraceThreads), 0, s, F, (int)level, cp,
+                hipLaunchKernelGGL(k_trace_closest<false>, dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp,
                                    fixed, ctx->d_status);
             }
         }
         {
-            LaunchTimer t(ctx, RTG_KERNEL_SHADE, s, timed);
+            LaunchTimer t(ctx, RTG_KERNEL_SHADE, st, timed);
             if (cnt)
-                hipLaunchKernelGGL(k_shade<true>, dim3(Gsmall), dim3(256), 0, s, F, (int)level, cp, fixed);
+                hipLaunchKernelGGL(k_shade<true>, dim3(gshade), dim3(256), 0, st, Fl, (int)level, cp, fixed);
             else
-                hipLaunchKernelGGL(k_shade<false>, dim3(Gsmall), dim3(256), 0, s, F, (int)level, cp, fixed);
+                hipLaunchKernelGGL(k_shade<false>, dim3(gshade), dim3(256), 0, st, Fl, (int)level, cp, fixed);
         }
-    }
-    {
-        LaunchTimer t(ctx, RTG_KERNEL_TRACE_OCCLUSION, s, timed);
+    };
+    auto occlusion = [&](const Frame& Fo, int part, unsigned g, hipStream_t st) {
+        LaunchTimer t(ctx, RTG_KERNEL_TRACE_OCCLUSION, st, timed);
         if (tune.refill_occlusion == 0) {
             if (cnt)
-                hipLaunchKernelGGL(k_trace_occlusion_rounds<true>, dim3(G), dim3(kTraceThreads), 0, s, F, ctx->d_status);
+                hipLaunchKernelGGL(k_trace_occlusion_rounds<true>, dim3(g), dim3(kTraceThreads), 0, st, Fo,
+                                   ctx->d_status, part);
             else
-                hipLaunchKernelGGL(k_trace_occlusion_rounds<false>, dim3(G), dim3(kTraceThreads), 0, s, F,
-                                   ctx->d_status);
+                hipLaunchKernelGGL(k_trace_occlusion_rounds<false>, dim3(g), dim3(kTraceThreads), 0, st, Fo,
+                                   ctx->d_status, part);
         } else if (cnt) {
-            hipLaunchKernelGGL(k_trace_occlusion<true>, dim3(G), dim3(kTraceThreads), 0, s, F, ctx->d_status);
+            hipLaunchKernelGGL(k_trace_occlusion<true>, dim3(g), dim3(kTraceThreads), 0, st, Fo, ctx->d_status, part);
         } else {
-            hipLaunchKernelGGL(k_trace_occlusion<false>, dim3(G), dim3(kTraceThreads), 0, s, F, ctx->d_status);
+            hipLaunchKernelGGL(k_trace_occlusion<false>, dim3(g), dim3(kTraceThreads), 0, st, Fo, ctx->d_status, part);
         }
+    };
+    level_step(F, 0, G, Gsmall, s);
+    const unsigned Gside = std::min<unsigned>((unsigned)tune.bounce_blocks, G / 2);
+    if (P.max_traces > 1 && Gside > 0) {
+        // The bounce levels hold few rays, so their launches are short of
+        // work and bound by the latency of their longest rays. They run on
+        // the side stream, on Gside persistent blocks, while the first
+        // level's shadow rays take the other blocks; the shadow rays the
+        // bounce levels add are traced on the side stream after them. The
+        // two grids use disjoint spill areas.
+        const unsigned Gmain = G - Gside;
+        Frame Fs = F;
+        Fs.spill = F.spill + (size_t)Gmain * kTraceThreads * kSpillDepth;
+        hipLaunchKernelGGL(k_shadow_snapshot, dim3(1), dim3(64), 0, s, a->d_counts);
+        hipEventRecord(a->fork, s);
+        hipStreamWaitEvent(a->side, a->fork, 0);
+        for (uint32_t level = 1; level < P.max_traces; ++level)
+            level_step(Fs, level, Gside, std::min(Gsmall, 4 * Gside), a->side);
+        occlusion(Fs, kPartRest, Gside, a->side);
+        hipEventRecord(a->join, a->side);
+        occlusion(F, kPartFirst, Gmain, s);
+        hipStreamWaitEvent(s, a->join, 0);
+    } else {
+        for (uint32_t level = 1; level < P.max_traces; ++level) level_step(F, level, G, Gsmall, s);
+        occlusion(F, kPartAll, G, s);
     }
     {
         LaunchTimer t(ctx, RTG_KERNEL_RESOLVE, s, timed);

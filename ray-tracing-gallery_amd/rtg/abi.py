@@ -266,15 +266,17 @@ class Tuning(C.Structure):
         ("refill_closest", C.c_uint32),
         ("refill_occlusion", C.c_uint32),
         ("path_order", C.c_uint32),
-        ("reserved", C.c_uint32 * 3),
+        ("bounce_blocks", C.c_uint32),
+        ("reserved", C.c_uint32 * 2),
     ]
 
     def as_dict(self):
         return {name: int(getattr(self, name)) for name, _ in self._fields_ if name != "reserved"}
 
     @staticmethod
-    def make(blocks_per_cu=4, feed_chunk=64, refill_closest=64, refill_occlusion=64, path_order=1):
-        return Tuning(blocks_per_cu, feed_chunk, refill_closest, refill_occlusion, path_order)
+    def make(blocks_per_cu=4, feed_chunk=64, refill_closest=64, refill_occlusion=64, path_order=1,
+             bounce_blocks=128):
+        return Tuning(blocks_per_cu, feed_chunk, refill_closest, refill_occlusion, path_order, bounce_blocks)
 
 
 assert C.sizeof(Tuning) == 32

@@ -127,6 +127,9 @@ SETTINGS = [
     dict(refill_closest=0, refill_occlusion=0),
     dict(refill_closest=32, refill_occlusion=48, path_order=0),
     dict(blocks_per_cu=1, feed_chunk=1024, refill_closest=1, refill_occlusion=1),
+    dict(bounce_blocks=0),
+    dict(bounce_blocks=1),
+    dict(blocks_per_cu=2, bounce_blocks=4096),
 ]
 
 
@@ -151,7 +154,7 @@ def test_tuning_never_changes_results(ctx, mini):
 def test_bad_tuning_is_refused(ctx):
     saved = ctx.tuning()
     for kw in [dict(blocks_per_cu=0), dict(blocks_per_cu=9), dict(feed_chunk=96), dict(refill_closest=65),
-               dict(path_order=2)]:
+               dict(path_order=2), dict(bounce_blocks=4097)]:
         with pytest.raises(Exception):
             ctx.set_tuning(abi.Tuning.make(**kw))
     assert ctx.tuning().as_dict() == saved.as_dict()

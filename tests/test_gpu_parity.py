@@ -247,6 +247,40 @@ def test_trace_image_denser_scene_frames(ctx):
     ds.destroy()
 
 
+def test_bounce_levels_beside_shadow_rays(ctx):
+    """The mirror bounce levels traced on a second stream beside the first
+    level's shadow rays (bounce_blocks > 0, down to one workgroup and up to
+    half the grid) give the same image, hits and traversal counts as every
+    level in sequence (bounce_blocks = 0), and equal the oracle."""
+    s = scenes.large_scene(width=160, height=90, n_objects=200, n_fences=16, n_mirrors=8)
+    ds = DeviceScene(ctx, s)
+    hs = oh.HostScene(s)
+    saved = ctx.tuning()
+    results = []
+    try:
+        for bb in (0, 1, 128, 4096):
+            ctx.set_tuning(abi.Tuning.make(bounce_blocks=bb))
+            ctx.set_counters(True)
+            rgba, rgb, ph = ds.trace_image()
+            torch.cuda.synchronize()
+            c = ctx.counters().as_dict()
+            ctx.set_counters(False)
+            results.append((rgba.cpu().numpy().view(np.uint32), rgb.cpu().numpy(),
+                            hits_to_numpy(ph).view(np.uint32), c))
+            if bb == 128:
+                _compare_image(ds, hs)
+    finally:
+        ctx.set_counters(False)
+        ctx.set_tuning(saved)
+    assert results[0][3]["bounce_rays"] > 0
+    for r in results[1:]:
+        assert np.array_equal(r[0], results[0][0])
+        assert np.array_equal(r[1], results[0][1])
+        assert np.array_equal(r[2], results[0][2])
+        assert r[3] == results[0][3]
+    ds.destroy()
+
+
 def hs_refit(hs, s):
     hs.refit([i.transform for i in s.instances])
     return hs
