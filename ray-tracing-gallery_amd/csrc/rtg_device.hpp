@@ -340,17 +340,27 @@ __device__ inline float4 sample_tex(const rtg_texture& t, const float* lut, floa
     return o;
 }
 
-__device__ inline float4 sample_index(const SceneView& S, uint32_t index, float u, float v) {
-    if (index >= S.texture_count) return make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-    const rtg_texture* tp = S.textures + index;
+// Descriptor of texture `index` of the table; a missing entry reads as an
+// empty texture, which samples as (0, 0, 0, 1).
+__device__ inline rtg_texture load_texture(const SceneView& S, uint32_t index) {
     rtg_texture t;
+    if (index >= S.texture_count) {
+        t.data = 0;
+        t.width = t.height = t.format = t.filter = 0;
+        return t;
+    }
+    const rtg_texture* tp = S.textures + index;
     t.data = gld_u64(&tp->data);
     uint4 rest = gld_u4(&tp->width);
     t.width = rest.x;
     t.height = rest.y;
     t.format = rest.z;
     t.filter = rest.w;
-    return sample_tex(t, S.srgb_lut, u, v);
+    return t;
+}
+
+__device__ inline float4 sample_index(const SceneView& S, uint32_t index, float u, float v) {
+    return sample_tex(load_texture(S, index), S.srgb_lut, u, v);
 }
 
 // ---------------------------------------------------------------------------

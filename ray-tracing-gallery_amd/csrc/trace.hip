@@ -590,19 +590,36 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
         }
         if (want_shadow) {
             const SceneView& S = J->scene;
-            for (uint32_t k = 0; k < want_shadow; ++k) {
-                // sample_blue_noise (:99-111) and animated_blue_noise (:116-120)
-                uint32_t o1x = k * 2u * 13u, o1y = k * 2u * 41u;
-                uint32_t o2x = (k * 2u + 1u) * 13u, o2y = (k * 2u + 1u) * 41u;
-                float4 t1 = sample_index(S, bn_index, (float)(px + o1x) / 64.0f, (float)(py + o1y) / 64.0f);
-                float4 t2 = sample_index(S, bn_index, (float)(px + o2x) / 64.0f, (float)(py + o2y) / 64.0f);
-                float anim = (float)(frame % 32u) * 0.618033988749f;
-                float rx = fractf(t1.x + anim), ry = fractf(t2.x + anim);
-                V3 sd = sample_dir_light(rx, ry, sun, sun_radius);
-                uint32_t slot = sbase + k;
-                F.sq.o[slot] = make_float4(sh_origin.x, sh_origin.y, sh_origin.z, 0.001f);
-                F.sq.d[slot] = make_float4(sd.x, sd.y, sd.z, 10000.0f);
-                F.sq.path[slot] = p;
+            // the blue-noise descriptor once, then the texels of four rays at a
+            // time before their stores, so that the loads overlap
+            const rtg_texture bn = load_texture(S, bn_index);
+            const float anim = (float)(frame % 32u) * 0.618033988749f;
+            for (uint32_t k0 = 0; k0 < want_shadow; k0 += 4) {
+                float n1[4], n2[4];
+    #pragma unroll
+                for (uint32_t j = 0; j < 4; ++j) {
+                    // sample_blue_noise (:99-111) and animated_blue_noise (:116-120)
+                    const uint32_t k = k0 + j;
+                    n1[j] = n2[j] = 0.0f;
+                    if (k < want_shadow) {
+                        uint32_t o1x = k * 2u * 13u, o1y = k * 2u * 41u;
+                        uint32_t o2x = (k * 2u + 1u) * 13u, o2y = (k * 2u + 1u) * 41u;
+                        n1[j] = sample_tex(bn, S.srgb_lut, (float)(px + o1x) / 64.0f, (float)(py + o1y) / 64.0f).x;
+                        n2[j] = sample_tex(bn, S.srgb_lut, (float)(px + o2x) / 64.0f, (float)(py + o2y) / 64.0f).x;
+                    }
+                }
+    #pragma unroll
+                for (uint32_t j = 0; j < 4; ++j) {
+                    const uint32_t k = k0 + j;
+                    if (k < want_shadow) {
+                        float rx = fractf(n1[j] + anim), ry = fractf(n2[j] + anim);
+                        V3 sd = sample_dir_light(rx, ry, sun, sun_radius);
+                        uint32_t slot = sbase + k;
+                        F.sq.o[slot] = make_float4(sh_origin.x, sh_origin.y, sh_origin.z, 0.001f);
+                        F.sq.d[slot] = make_float4(sd.x, sd.y, sd.z, 10000.0f);
+                        F.sq.path[slot] = p;
+                    }
+                }
             }
             n_shadow += want_shadow;
         }
