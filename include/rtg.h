@@ -344,6 +344,20 @@ typedef struct rtg_trace_counters {
     uint64_t occ_triangle_tests;
     uint64_t occ_instance_visits;
     uint64_t occ_anyhit_calls;
+    /* of the shadow-ray and occlusion counts above, those traced on the
+       bounce stream beside the first level's shadow rays
+       (rtg_tuning.bounce_blocks; zero when every level runs in sequence) */
+    uint64_t side_shadow_rays;
+    uint64_t side_occ_node_visits;
+    uint64_t side_occ_triangle_tests;
+    uint64_t side_occ_instance_visits;
+    uint64_t side_occ_anyhit_calls;
+    /* of the closest-hit counts above, those of the bounce levels traced on
+       the bounce stream (their rays are bounce_rays) */
+    uint64_t side_node_visits;
+    uint64_t side_triangle_tests;
+    uint64_t side_instance_visits;
+    uint64_t side_anyhit_calls;
 } rtg_trace_counters;
 /* Enables traversal counters (slower kernels) for subsequent traces. */
 int rtg_set_counters(rtg_ctx* ctx, int enable);
@@ -367,7 +381,9 @@ int rtg_get_status(rtg_ctx* ctx, uint32_t* bits, int clear);
 #define RTG_KERNEL_RESOLVE 4u
 #define RTG_KERNEL_REFIT 5u
 #define RTG_KERNEL_BATCH_GRAPH 6u /* one replay of an rtg_batch graph (all its kernels) */
-#define RTG_KERNEL_COUNT 7u
+#define RTG_KERNEL_BOUNCE_SIDE 7u /* the bounce levels and their shadow rays on the bounce stream,
+                                     beside the first level's shadow rays (one span per frame) */
+#define RTG_KERNEL_COUNT 8u
 int rtg_set_timing(rtg_ctx* ctx, int enable);
 int rtg_get_kernel_time(rtg_ctx* ctx, uint32_t kind, double* total_ms, uint64_t* launches);
 

@@ -170,6 +170,16 @@ extern "C" int rtg_get_counters(rtg_ctx* ctx, rtg_trace_counters* out) {
     RTG_REQUIRE(ctx && out, RTG_ERROR_INVALID_ARGUMENT, "rtg_get_counters: null argument");
     RTG_HIP(hipDeviceSynchronize());
     RTG_HIP(hipMemcpy(out, ctx->d_counters, sizeof(*out), hipMemcpyDeviceToHost));
+    // the device keeps the bounce stream's traversal counts apart; the
+    // totals include them
+    out->occ_node_visits += out->side_occ_node_visits;
+    out->occ_triangle_tests += out->side_occ_triangle_tests;
+    out->occ_instance_visits += out->side_occ_instance_visits;
+    out->occ_anyhit_calls += out->side_occ_anyhit_calls;
+    out->node_visits += out->side_node_visits;
+    out->triangle_tests += out->side_triangle_tests;
+    out->instance_visits += out->side_instance_visits;
+    out->anyhit_calls += out->side_anyhit_calls;
     return RTG_OK;
 }
 

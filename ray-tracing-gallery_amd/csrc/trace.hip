@@ -105,6 +105,7 @@ struct Frame {
     int refill_closest;       // idle lanes that trigger a refill (0: rounds kernel)
     int refill_occlusion;
     uint32_t feed_chunk;      // queue entries a wave takes per atomic
+    uint32_t side;            // 1 on the bounce stream (counts its shadow rays apart)
 };
 
 __device__ inline uint32_t find_job(const JobDev* jobs, uint32_t njobs, uint64_t p, bool by_pixel) {
@@ -141,10 +142,15 @@ __device__ inline void wave_add_counter(uint64_t* c, uint32_t v) {
 }
 
 // Traversal counters of rtg_trace_counters: slots 3..6 for closest-hit
-// queries, 7..10 for occlusion queries.
+// queries, 7..10 for occlusion queries; the bounce stream counts its shadow
+// rays in slot 11 (and slot 2), their occlusion queries in 12..15 and its
+// closest-hit queries in 16..19 (added to 7..10 and 3..6 when read).
 constexpr int kCountersClosest = 3;
 constexpr int kCountersOcclusion = 7;
-constexpr int kCounterSlots = 11;
+constexpr int kCounterSideShadow = 11;
+constexpr int kCountersOcclusionSide = 12;
+constexpr int kCountersClosestSide = 16;
+constexpr int kCounterSlots = 20;
 
 __device__ inline void flush_counters(uint64_t* counters, const TravCounters& c, int base) {
     wave_add_counter(&counters[base + 0], c.nodes);
@@ -280,7 +286,8 @@ __global__ __launch_bounds__(256) void k_raygen(Frame F, bool count) {
 }
 
 // Closest-hit traversal of trace level b (count from the device unless fixed).
-template <bool COUNT>
+// SIDE instances run the bounce levels on the bounce stream.
+template <bool COUNT, bool SIDE>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int level, const uint32_t* count_ptr,
                                                                  uint32_t fixed_count, uint32_t* status) {
     __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
@@ -328,12 +335,12 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int le
             }
         }
     }
-    if (COUNT) flush_counters(F.counters, cnt, kCountersClosest);
+    if (COUNT) flush_counters(F.counters, cnt, SIDE ? kCountersClosestSide : kCountersClosest);
 }
 
 // Non-persistent variant: lanes take rays in grid-stride rounds and run each
 // to completion (a wave waits for its slowest lane).
-template <bool COUNT>
+template <bool COUNT, bool SIDE>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_closest_rounds(Frame F, int level, const uint32_t* count_ptr,
                                                                         uint32_t fixed_count, uint32_t* status) {
     __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
@@ -362,7 +369,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest_rounds(Frame F,
             F.hit_b[i] = make_float2(h.u, h.v);
         }
     }
-    if (COUNT) flush_counters(F.counters, cnt, kCountersClosest);
+    if (COUNT) flush_counters(F.counters, cnt, SIDE ? kCountersClosestSide : kCountersClosest);
 }
 
 __device__ inline V3 ld3(uint64_t base, uint32_t i) {
@@ -627,6 +634,7 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
     if (COUNT) {
         wave_add_counter(&F.counters[1], n_bounce);
         wave_add_counter(&F.counters[2], n_shadow);
+        if (F.side) wave_add_counter(&F.counters[kCounterSideShadow], n_shadow);
     }
 }
 
@@ -642,7 +650,7 @@ __device__ inline void shadow_part(const Frame& F, int part, uint32_t& lo, uint3
     feed = &F.counts[part == kPartRest ? kFeedShadowRest : kFeedShadow];
 }
 
-template <bool COUNT>
+template <bool COUNT, bool SIDE>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion(Frame F, uint32_t* status, int part) {
     __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
     uint32_t lo, hi;
@@ -691,10 +699,10 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion(Frame F, uint
             }
         }
     }
-    if (COUNT) flush_counters(F.counters, cnt, kCountersOcclusion);
+    if (COUNT) flush_counters(F.counters, cnt, SIDE ? kCountersOcclusionSide : kCountersOcclusion);
 }
 
-template <bool COUNT>
+template <bool COUNT, bool SIDE>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion_rounds(Frame F, uint32_t* status, int part) {
     __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
     uint32_t lo, hi;
@@ -724,7 +732,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion_rounds(Frame 
             if (res == 0) atomicAdd(&F.vis[p], 1u);
         }
     }
-    if (COUNT) flush_counters(F.counters, cnt, kCountersOcclusion);
+    if (COUNT) flush_counters(F.counters, cnt, SIDE ? kCountersOcclusionSide : kCountersOcclusion);
 }
 
 // linear_to_srgb (lib.rs:84-92)
@@ -1058,6 +1066,7 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
     F.refill_closest = tune.refill_closest;
     F.refill_occlusion = tune.refill_occlusion;
     F.feed_chunk = (uint32_t)tune.feed_chunk;
+    F.side = 0;
     const unsigned G = (unsigned)trace_grid(ctx, tune);
     const unsigned Gsmall = (unsigned)std::min<uint64_t>((P.paths + 255) / 256, (uint64_t)ctx->cu_count * 16);
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, a->d_counts, ctx->d_counters, cnt);
@@ -1067,50 +1076,56 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
     }
     // trace level `level` (closest hits, then the hit and miss stages) on
     // `st` with `g` persistent blocks
-    auto level_step = [&](const Frame& Fl, uint32_t level, unsigned g, unsigned gshade, hipStream_t st) {
+    auto level_step = [&](const Frame& Fl, uint32_t level, unsigned g, unsigned gshade, hipStream_t st, bool tm) {
         const uint32_t* cp = level == 0 ? nullptr : a->d_counts + level;
         uint32_t fixed = level == 0 ? (uint32_t)P.paths : 0u;
         {
-            LaunchTimer t(ctx, RTG_KERNEL_TRACE_CLOSEST, st, timed);
+            LaunchTimer t(ctx, RTG_KERNEL_TRACE_CLOSEST, st, tm);
+            const bool side = Fl.side != 0;
             if (tune.refill_closest == 0) {
-                if (cnt)
-                    hipLaunchKernelGGL(k_trace_closest_rounds<true>, dim3(g), dim3(kTraceThreads), 0, st, Fl,
-                                       (int)level, cp, fixed, ctx->d_status);
+                if (side)
+                    hipLaunchKernelGGL((cnt ? k_trace_closest_rounds<true, true> : k_trace_closest_rounds<false, true>),
+                                       dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, ctx->d_status);
                 else
-                    hipLaunchKernelGGL(k_trace_closest_rounds<false>, dim3(g), dim3(kTraceThreads), 0, st, Fl,
-                                       (int)level, cp, fixed, ctx->d_status);
-            } else if (cnt) {
-                hipLaunchKernelGGL(k_trace_closest<true>, dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp,
-                                   fixed, ctx->d_status);
+                    hipLaunchKernelGGL((cnt ? k_trace_closest_rounds<true, false> : k_trace_closest_rounds<false, false>),
+                                       dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, ctx->d_status);
+            } else if (side) {
+                hipLaunchKernelGGL((cnt ? k_trace_closest<true, true> : k_trace_closest<false, true>), dim3(g),
+                                   dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, ctx->d_status);
             } else {
-                hipLaunchKernelGGL(k_trace_closest<false>, dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp,
-                                   fixed, ctx->d_status);
+                hipLaunchKernelGGL((cnt ? k_trace_closest<true, false> : k_trace_closest<false, false>), dim3(g),
+                                   dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, ctx->d_status);
             }
         }
         {
-            LaunchTimer t(ctx, RTG_KERNEL_SHADE, st, timed);
+            LaunchTimer t(ctx, RTG_KERNEL_SHADE, st, tm);
             if (cnt)
                 hipLaunchKernelGGL(k_shade<true>, dim3(gshade), dim3(256), 0, st, Fl, (int)level, cp, fixed);
             else
                 hipLaunchKernelGGL(k_shade<false>, dim3(gshade), dim3(256), 0, st, Fl, (int)level, cp, fixed);
         }
     };
-    auto occlusion = [&](const Frame& Fo, int part, unsigned g, hipStream_t st) {
-        LaunchTimer t(ctx, RTG_KERNEL_TRACE_OCCLUSION, st, timed);
+    // occlusion rays of `part` of the shadow queue; SIDE kernels count into
+    // the bounce stream's counter slots
+    auto occlusion = [&](const Frame& Fo, int part, unsigned g, hipStream_t st, bool tm) {
+        LaunchTimer t(ctx, RTG_KERNEL_TRACE_OCCLUSION, st, tm);
+        const bool side = part == kPartRest;
         if (tune.refill_occlusion == 0) {
-            if (cnt)
-                hipLaunchKernelGGL(k_trace_occlusion_rounds<true>, dim3(g), dim3(kTraceThreads), 0, st, Fo,
-                                   ctx->d_status, part);
+            if (side)
+                hipLaunchKernelGGL((cnt ? k_trace_occlusion_rounds<true, true> : k_trace_occlusion_rounds<false, true>),
+                                   dim3(g), dim3(kTraceThreads), 0, st, Fo, ctx->d_status, part);
             else
-                hipLaunchKernelGGL(k_trace_occlusion_rounds<false>, dim3(g), dim3(kTraceThreads), 0, st, Fo,
-                                   ctx->d_status, part);
-        } else if (cnt) {
-            hipLaunchKernelGGL(k_trace_occlusion<true>, dim3(g), dim3(kTraceThreads), 0, st, Fo, ctx->d_status, part);
+                hipLaunchKernelGGL((cnt ? k_trace_occlusion_rounds<true, false> : k_trace_occlusion_rounds<false, false>),
+                                   dim3(g), dim3(kTraceThreads), 0, st, Fo, ctx->d_status, part);
+        } else if (side) {
+            hipLaunchKernelGGL((cnt ? k_trace_occlusion<true, true> : k_trace_occlusion<false, true>), dim3(g),
+                               dim3(kTraceThreads), 0, st, Fo, ctx->d_status, part);
         } else {
-            hipLaunchKernelGGL(k_trace_occlusion<false>, dim3(g), dim3(kTraceThreads), 0, st, Fo, ctx->d_status, part);
+            hipLaunchKernelGGL((cnt ? k_trace_occlusion<true, false> : k_trace_occlusion<false, false>), dim3(g),
+                               dim3(kTraceThreads), 0, st, Fo, ctx->d_status, part);
         }
     };
-    level_step(F, 0, G, Gsmall, s);
+    level_step(F, 0, G, Gsmall, s, timed);
     const unsigned Gside = std::min<unsigned>((unsigned)tune.bounce_blocks, G / 2);
     if (P.max_traces > 1 && Gside > 0) {
         // The bounce levels hold few rays, so their launches are short of
@@ -1122,18 +1137,23 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
         const unsigned Gmain = G - Gside;
         Frame Fs = F;
         Fs.spill = F.spill + (size_t)Gmain * kTraceThreads * kSpillDepth;
+        Fs.side = 1;
         hipLaunchKernelGGL(k_shadow_snapshot, dim3(1), dim3(64), 0, s, a->d_counts);
         hipEventRecord(a->fork, s);
         hipStreamWaitEvent(a->side, a->fork, 0);
-        for (uint32_t level = 1; level < P.max_traces; ++level)
-            level_step(Fs, level, Gside, std::min(Gsmall, 4 * Gside), a->side);
-        occlusion(Fs, kPartRest, Gside, a->side);
+        {
+            // timed as one span: its kernels share the GPU with the shadow rays
+            LaunchTimer t(ctx, RTG_KERNEL_BOUNCE_SIDE, a->side, timed);
+            for (uint32_t level = 1; level < P.max_traces; ++level)
+                level_step(Fs, level, Gside, std::min(Gsmall, 4 * Gside), a->side, false);
+            occlusion(Fs, kPartRest, Gside, a->side, false);
+        }
         hipEventRecord(a->join, a->side);
-        occlusion(F, kPartFirst, Gmain, s);
+        occlusion(F, kPartFirst, Gmain, s, timed);
         hipStreamWaitEvent(s, a->join, 0);
     } else {
-        for (uint32_t level = 1; level < P.max_traces; ++level) level_step(F, level, G, Gsmall, s);
-        occlusion(F, kPartAll, G, s);
+        for (uint32_t level = 1; level < P.max_traces; ++level) level_step(F, level, G, Gsmall, s, timed);
+        occlusion(F, kPartAll, G, s, timed);
     }
     {
         LaunchTimer t(ctx, RTG_KERNEL_RESOLVE, s, timed);

@@ -47,7 +47,8 @@ RTG_STATUS_BAD_INDEX = 0x1
 RTG_STATUS_UNKNOWN_BLAS = 0x2
 RTG_STATUS_STACK_OVERFLOW = 0x4
 
-RTG_KERNEL_NAMES = ["raygen", "trace_closest", "shade", "trace_occlusion", "resolve", "refit", "batch_graph"]
+RTG_KERNEL_NAMES = ["raygen", "trace_closest", "shade", "trace_occlusion", "resolve", "refit", "batch_graph",
+                    "bounce_side"]
 RTG_KERNEL_RAYGEN = 0
 RTG_KERNEL_TRACE_CLOSEST = 1
 RTG_KERNEL_SHADE = 2
@@ -55,6 +56,7 @@ RTG_KERNEL_TRACE_OCCLUSION = 3
 RTG_KERNEL_RESOLVE = 4
 RTG_KERNEL_REFIT = 5
 RTG_KERNEL_BATCH_GRAPH = 6
+RTG_KERNEL_BOUNCE_SIDE = 7
 
 
 class Instance(C.Structure):
@@ -207,6 +209,15 @@ class TraceCounters(C.Structure):
         ("occ_triangle_tests", C.c_uint64),
         ("occ_instance_visits", C.c_uint64),
         ("occ_anyhit_calls", C.c_uint64),
+        ("side_shadow_rays", C.c_uint64),
+        ("side_occ_node_visits", C.c_uint64),
+        ("side_occ_triangle_tests", C.c_uint64),
+        ("side_occ_instance_visits", C.c_uint64),
+        ("side_occ_anyhit_calls", C.c_uint64),
+        ("side_node_visits", C.c_uint64),
+        ("side_triangle_tests", C.c_uint64),
+        ("side_instance_visits", C.c_uint64),
+        ("side_anyhit_calls", C.c_uint64),
     ]
 
     def as_dict(self):

@@ -273,11 +273,15 @@ def test_bounce_levels_beside_shadow_rays(ctx):
         ctx.set_counters(False)
         ctx.set_tuning(saved)
     assert results[0][3]["bounce_rays"] > 0
+    totals = lambda c: {k: v for k, v in c.items() if not k.startswith("side_")}
+    assert not any(v for k, v in results[0][3].items() if k.startswith("side_"))
+    assert results[2][3]["side_occ_node_visits"] > 0 and results[2][3]["side_node_visits"] > 0
+    assert results[2][3]["side_shadow_rays"] > 0
     for r in results[1:]:
         assert np.array_equal(r[0], results[0][0])
         assert np.array_equal(r[1], results[0][1])
         assert np.array_equal(r[2], results[0][2])
-        assert r[3] == results[0][3]
+        assert totals(r[3]) == totals(results[0][3])
     ds.destroy()
 
 
