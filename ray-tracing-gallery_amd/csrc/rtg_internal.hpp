@@ -146,7 +146,8 @@ size_t frame_bytes(uint64_t paths, uint64_t shadow, uint32_t njobs);
 // (the context's own arena if null); check_tlas validates each job's TLAS
 // against the context's structures.
 int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStream_t stream, FrameArena* a,
-               bool check_tlas);
+               bool check_tlas,
+               const Tuning* tune = nullptr);
 // A trace step split in its host half and its device half.
 struct TracePlan {
     uint64_t paths = 0, pixels = 0, shadow = 0;

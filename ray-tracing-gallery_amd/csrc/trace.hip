@@ -961,6 +961,12 @@ static int ensure_frame(rtg_ctx* ctx, FrameArena* a, uint64_t paths, uint64_t sh
         a->h_jobs_cap = njobs;
     }
     if (!a->staging_free) RTG_HIP(hipEventCreateWithFlags(&a->staging_free, hipEventDisableTiming));
+    return RTG_OK;
+}
+
+// The bounce stream of an arena and its fork/join events, created on first
+// use (a stream costs a hardware-queue share even while idle).
+static int ensure_side(FrameArena* a) {
     if (!a->side) RTG_HIP(hipStreamCreateWithFlags(&a->side, hipStreamNonBlocking));
     if (!a->fork) RTG_HIP(hipEventCreateWithFlags(&a->fork, hipEventDisableTiming));
     if (!a->join) RTG_HIP(hipEventCreateWithFlags(&a->join, hipEventDisableTiming));
@@ -1010,6 +1016,10 @@ int plan_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, FrameAren
     RTG_HIP(hipSetDevice(ctx->device));
     int st = ensure_frame(ctx, a, P.paths, P.shadow, njobs, tune);
     if (st != RTG_OK) return st;
+    if (P.max_traces > 1 && tune.bounce_blocks > 0) {
+        st = ensure_side(a);
+        if (st != RTG_OK) return st;
+    }
     // the pinned job table is reused: wait until the previous upload left it
     RTG_HIP(hipEventSynchronize(a->staging_free));
     uint64_t sb = 0, pb = 0;
@@ -1163,14 +1173,15 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
 }
 
 int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStream_t s, FrameArena* a,
-               bool check_tlas) {
+               bool check_tlas, const Tuning* tune) {
     if (a == nullptr) a = arena(ctx);
+    const Tuning& t = tune ? *tune : ctx->tune;
     TracePlan P;
-    int st = plan_jobs(ctx, jobs, njobs, a, check_tlas, ctx->tune, P);
+    int st = plan_jobs(ctx, jobs, njobs, a, check_tlas, t, P);
     if (st != RTG_OK) return st;
     st = upload_jobs(a, P, s);
     if (st != RTG_OK) return st;
-    enqueue_trace(ctx, a, P, ctx->tune, ctx->counters_enabled, true, s);
+    enqueue_trace(ctx, a, P, t, ctx->counters_enabled, true, s);
     RTG_HIP(hipGetLastError());
     return RTG_OK;
 }
