@@ -4,9 +4,11 @@ soft-shadow rays per hit, up to 3 traces per path, a TLAS refit every frame).
 
 A step is one frame: TLAS refit with that frame's instance transforms
 (already resident in HBM) + the full trace (raygen, closest-hit traversal,
-shading, shadow occlusion, resolve). Rays counted = primary + bounce +
-shadow rays actually traced, from the device counters of a replay of the same
-frames. Prints one JSON line.
+shading, shadow occlusion, resolve). Two frames are in flight, each with its
+own TLAS copy, stream and output images, as the reference's frame loop keeps
+them (src/main.rs:600-668); --frames-in-flight 1 traces one after another.
+Rays counted = primary + bounce + shadow rays actually traced, from the
+device counters of a replay of the same frames. Prints one JSON line.
 """
 import argparse
 import json
@@ -349,6 +351,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--frames-in-flight", type=int, default=2,
+                    help="frames traced concurrently on their own streams and TLAS copies, as the reference's "
+                         "frame loop keeps them (src/main.rs:600-668); 1 = one frame after another")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity check")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream benchmark")
     ap.add_argument("--serve-requests", type=int, default=160)
@@ -387,14 +392,20 @@ def main():
     ds = DeviceScene(ctx, scene, "cuda:0")
     W, K = args.warmup, args.steps
     bufs = [ds.instance_buffer(scene.transforms_for_frame(f)) for f in range(W + K)]
-    outputs = ds.make_outputs(hits=False)
+    # frames in flight: frame f uses slot f % FIF (a TLAS copy, a stream and
+    # output images of its own); the library keeps one frame state per stream
+    FIF = max(1, args.frames_in_flight)
+    slot_tlas = [ds.clone_tlas() for _ in range(FIF)] if FIF > 1 else [None]
+    slot_stream = [torch.cuda.Stream() for _ in range(FIF)] if FIF > 1 else [None]
+    slot_out = [ds.make_outputs(hits=False) for _ in range(FIF)]
     torch.cuda.synchronize()
     log(f"setup {time.perf_counter() - t_setup:.1f}s: {len(scene.instances)} instances, "
-        f"{scene.instanced_triangles} instanced triangles, {args.width}x{args.height}")
+        f"{scene.instanced_triangles} instanced triangles, {args.width}x{args.height}, {FIF} frames in flight")
 
     def frame(f):
-        ds.refit(bufs[f])
-        ds.trace_image(frame_index=f, outputs=outputs)
+        k = f % FIF
+        ds.refit(bufs[f], tlas=slot_tlas[k], stream=slot_stream[k])
+        ds.trace_image(frame_index=f, outputs=slot_out[k], stream=slot_stream[k], tlas=slot_tlas[k])
 
     for f in range(W):
         frame(f)
@@ -446,7 +457,7 @@ def main():
                                      "TLAS refit every frame",
                          "width": args.width, "height": args.height, "instances": len(scene.instances),
                          "instanced_triangles": scene.instanced_triangles, "spp": 1,
-                         "rays_per_frame": int(rays / K), "seed": scene.seed},
+                         "rays_per_frame": int(rays / K), "frames_in_flight": FIF, "seed": scene.seed},
               "roofline": roofline, "kernel_ms_per_frame": kernel_ms,
               "traversal": {"nodes_per_closest_ray": round(tot["node_visits"] / max(1, tot["primary_rays"] +
                                                                                     tot["bounce_rays"]), 2),

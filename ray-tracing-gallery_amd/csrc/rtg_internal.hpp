@@ -121,7 +121,8 @@ struct rtg_ctx {
     std::vector<rtg::TimedLaunch> pending;
     double kernel_ms[RTG_KERNEL_COUNT] = {};
     uint64_t kernel_launches[RTG_KERNEL_COUNT] = {};
-    rtg::FrameArena* frame = nullptr;
+    // frame state of the context's own calls, one per stream they are made on
+    std::map<void*, rtg::FrameArena*> frames;
     rtg::Tuning tune;
 };
 
@@ -158,7 +159,8 @@ struct TracePlan {
 // The arena's spill area is sized for `tune`.
 int plan_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, FrameArena* a, bool check_tlas,
               const Tuning& tune, TracePlan& P);
-// The frame state of the context's own calls (created on first use).
+// The frame state of the context's own calls on the null stream (created on
+// first use).
 FrameArena* context_arena(rtg_ctx* ctx);
 // Device pointers of an arena's buffers (a captured graph keeps them).
 void arena_pointers(const FrameArena* a, const void* out[3]);

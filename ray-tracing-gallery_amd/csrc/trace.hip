@@ -863,8 +863,8 @@ void arena_pointers(const FrameArena* a, const void* out[3]) {
 }
 
 void free_frame_arena(rtg_ctx* ctx) {
-    delete_frame_arena(ctx->frame);
-    ctx->frame = nullptr;
+    for (auto& kv : ctx->frames) delete_frame_arena(kv.second);
+    ctx->frames.clear();
 }
 
 // Device bytes of the frame state of `paths` paths and `shadow` shadow rays.
@@ -899,11 +899,14 @@ static int ensure_spill(rtg_ctx* ctx, FrameArena* a, const Tuning& tune) {
     return a->feed.ensure(256);
 }
 
-static FrameArena* arena(rtg_ctx* ctx) {
-    if (!ctx->frame) ctx->frame = new FrameArena();
-    return ctx->frame;
+// Frame state of the context's calls on stream `s`: calls on different
+// streams (frames in flight) never share queues, counters or spill areas.
+static FrameArena* arena(rtg_ctx* ctx, hipStream_t s) {
+    FrameArena*& a = ctx->frames[reinterpret_cast<void*>(s)];
+    if (!a) a = new FrameArena();
+    return a;
 }
-FrameArena* context_arena(rtg_ctx* ctx) { return arena(ctx); }
+FrameArena* context_arena(rtg_ctx* ctx) { return arena(ctx, nullptr); }
 
 static int ensure_frame(rtg_ctx* ctx, FrameArena* a, uint64_t paths, uint64_t shadow, uint32_t njobs,
                         const Tuning& tune) {
@@ -1174,7 +1177,7 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
 
 int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStream_t s, FrameArena* a,
                bool check_tlas, const Tuning* tune) {
-    if (a == nullptr) a = arena(ctx);
+    if (a == nullptr) a = arena(ctx, s);
     const Tuning& t = tune ? *tune : ctx->tune;
     TracePlan P;
     int st = plan_jobs(ctx, jobs, njobs, a, check_tlas, t, P);
@@ -1224,7 +1227,7 @@ extern "C" int rtg_trace_rays(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindi
     if (count == 0) return RTG_OK;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     RTG_HIP(hipSetDevice(ctx->device));
-    FrameArena* a = arena(ctx);
+    FrameArena* a = arena(ctx, s);
     if (ensure_spill(ctx, a, ctx->tune) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
     SceneView S;
     S.models = scene ? reinterpret_cast<const rtg_model_info*>(scene->model_info) : nullptr;
