@@ -76,3 +76,27 @@ def test_frame_matches_oracle(ctx, animated):
     assert np.array_equal(hits_to_numpy(o[2]).view(np.uint32), hits.view(np.uint32))
     assert np.array_equal(o[0].cpu().numpy().view(np.uint32), rgba)
     assert np.abs(o[1].cpu().numpy().reshape(-1, 3) - rgb).max() <= 1e-4
+
+
+def test_plain_calls_on_two_streams_run_as_frames_in_flight(ctx, animated):
+    """rtg_trace_image on two streams at once (a TLAS copy each): the context
+    keeps one frame state per stream, so each frame equals the same frame
+    traced alone."""
+    s, ds = animated
+    streams = [torch.cuda.Stream() for _ in range(2)]
+    clones = [ds.clone_tlas() for _ in range(2)]
+    bufs = [ds.instance_buffer(s.transforms_for_frame(f)) for f in range(6)]
+    outs = [ds.make_outputs() for _ in range(6)]
+    torch.cuda.synchronize()
+    for f in range(6):
+        k = f % 2
+        ds.refit(bufs[f], tlas=clones[k], stream=streams[k])
+        ds.trace_image(frame_index=f, outputs=outs[f], stream=streams[k], tlas=clones[k])
+    torch.cuda.synchronize()
+    got = [tuple(t.cpu().numpy().copy() for t in o) for o in outs]
+    for c in clones:
+        load().rtg_tlas_destroy(ctx.ptr, c)
+    for f in range(6):
+        want = _plain(ctx, ds, f)
+        for a, b in zip(got[f], want):
+            assert np.array_equal(a, b), f
