@@ -36,12 +36,19 @@
 extern "C" {
 #endif
 
-/* Builds a BLAS blob in host memory (free with orc_free). */
-int orc_blas_build(const rtg_geometry* geoms, uint32_t geom_count, void** out_blob,
+/* Builds a BLAS blob in host memory (free with orc_free). flags: the
+ * RTG_BUILD_* flags of rtg_blas_build (PREFER_FAST_TRACE adds treelet
+ * restructuring to the LBVH). */
+int orc_blas_build(const rtg_geometry* geoms, uint32_t geom_count, uint32_t flags, void** out_blob,
                    uint64_t* out_size);
 /* Builds a TLAS blob; instance BLAS references are orc BLAS blob pointers. */
-int orc_tlas_build(const rtg_instance* instances, uint32_t count, void** out_blob,
+int orc_tlas_build(const rtg_instance* instances, uint32_t count, uint32_t flags, void** out_blob,
                    uint64_t* out_size);
+/* Test hook: the binary tree a build over n >= 2 primitive boxes (6 floats
+ * each: lo xyz, hi xyz) collapses, after its treelet passes: internal node i
+ * has children lc[i], rc[i] (bit 31 set: leaf k, the primitive order[k]). */
+int orc_debug_binary_tree(const float* boxes, uint32_t n, uint32_t flags, uint32_t* lc, uint32_t* rc,
+                          uint32_t* order);
 /* In-place refit of a TLAS blob with new instance records (same count). */
 int orc_tlas_refit(void* tlas_blob, const rtg_instance* instances, uint32_t count);
 void orc_free(void* blob);

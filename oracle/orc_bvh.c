@@ -9,7 +9,9 @@
  * The driver's own BVH is closed; the method restated here is the linear BVH
  * of Karras 2012 (HPG, "Maximizing Parallelism in the Construction of BVHs,
  * Octrees, and k-d Trees"): 30-bit Morton codes of primitive box centres,
- * a stable sort, the radix-tree split search, bottom-up box merging.
+ * a stable sort, the radix-tree split search, bottom-up box merging; with
+ * RTG_BUILD_PREFER_FAST_TRACE, treelet restructuring (Karras & Aila 2013)
+ * of the binary tree before its collapse into 4-wide nodes.
  */
 #include <float.h>
 #include <math.h>
@@ -232,8 +234,116 @@ static int expand_slots(const bintree* t, uint32_t r, uint32_t leaf_max, uint32_
     return m;
 }
 
-/* Builds the wide tree over n primitive boxes. */
-static int wide_build(const box3* prim_boxes, uint32_t n, uint32_t leaf_max, wtree* out) {
+/* Treelet restructuring of the binary tree (Karras & Aila 2013, HPG, "Fast
+ * Parallel Construction of High-Quality Bounding Volume Hierarchies",
+ * section 4), with single-primitive leaves. The treelet of internal node r:
+ * r's two children, then, while fewer than ORC_TREELET_LEAVES, the treelet
+ * leaf with the largest area that is an internal node replaced by its left
+ * child in place and its right child appended. Its topology is replaced by
+ * the one that minimises the summed surface area of its internal nodes
+ * (the SAH cost with fixed leaves), found by dynamic programming over the
+ * subsets of treelet leaves: cost(S) = area(S) + min over partitions (P, S\P),
+ * P holding S's lowest leaf, of cost(P) + cost(S\P), partitions tried in
+ * decreasing order of P with the first minimum kept. The new internal nodes
+ * reuse the treelet's node numbers: r for the whole set, then the expanded
+ * nodes in expansion order, handed out as subsets are split from a stack
+ * (P's subtree first). */
+#define ORC_TREELET_LEAVES 7
+
+static box3 bin_box(const box3* ibox, const box3* leaf_boxes, uint32_t c) {
+    return (c & ORC_LEAF_BIT) ? leaf_boxes[c & ~ORC_LEAF_BIT] : ibox[c];
+}
+
+static void treelet_optimize(bintree* t, box3* ibox, const box3* leaf_boxes, uint32_t r) {
+    uint32_t leaf[ORC_TREELET_LEAVES], inner[ORC_TREELET_LEAVES];
+    int nl = 2, ni = 0;
+    leaf[0] = t->lc[r];
+    leaf[1] = t->rc[r];
+    while (nl < ORC_TREELET_LEAVES) {
+        int best = -1;
+        float best_area = 0.0f;
+        for (int k = 0; k < nl; ++k) {
+            uint32_t c = leaf[k];
+            if (c & ORC_LEAF_BIT) continue;
+            if (best < 0 || t->area[c] > best_area) {
+                best = k;
+                best_area = t->area[c];
+            }
+        }
+        if (best < 0) break;
+        uint32_t c = leaf[best];
+        inner[ni++] = c;
+        leaf[best] = t->lc[c];
+        leaf[nl++] = t->rc[c];
+    }
+    if (nl < 3) return;
+    const uint32_t full = (1u << nl) - 1u;
+    float cost[1 << ORC_TREELET_LEAVES];
+    uint32_t split[1 << ORC_TREELET_LEAVES];
+    for (uint32_t S = 1; S <= full; ++S) {
+        if ((S & (S - 1)) == 0) {
+            cost[S] = 0.0f;
+            split[S] = 0;
+            continue;
+        }
+        box3 b = box_empty();
+        for (int k = 0; k < nl; ++k)
+            if (S & (1u << k)) b = box_union(b, bin_box(ibox, leaf_boxes, leaf[k]));
+        const uint32_t low = S & (0u - S);
+        float best = 0.0f;
+        uint32_t bp = 0;
+        for (uint32_t P = (S - 1u) & S; P != 0; P = (P - 1u) & S) {
+            if (!(P & low)) continue;
+            float c = cost[P] + cost[S ^ P];
+            if (bp == 0 || c < best) {
+                best = c;
+                bp = P;
+            }
+        }
+        cost[S] = box_area(b) + best;
+        split[S] = bp;
+    }
+    /* topology: subsets split from a stack, node numbers handed out in order */
+    uint32_t stk_set[ORC_TREELET_LEAVES], stk_id[ORC_TREELET_LEAVES], order[ORC_TREELET_LEAVES];
+    int sp = 0, no = 0, next = 0;
+    stk_set[sp] = full;
+    stk_id[sp++] = r;
+    while (sp > 0) {
+        --sp;
+        uint32_t S = stk_set[sp], id = stk_id[sp];
+        order[no++] = id;
+        uint32_t part[2] = {split[S], S ^ split[S]};
+        uint32_t ref[2];
+        for (int h = 0; h < 2; ++h)
+            ref[h] = (part[h] & (part[h] - 1u)) ? inner[next++] : leaf[__builtin_ctz(part[h])];
+        t->lc[id] = ref[0];
+        t->rc[id] = ref[1];
+        for (int h = 1; h >= 0; --h)
+            if (part[h] & (part[h] - 1u)) {
+                stk_set[sp] = part[h];
+                stk_id[sp++] = ref[h];
+            }
+    }
+    /* boxes, areas and leaf counts, children before parents */
+    for (int k = no - 1; k >= 0; --k) {
+        uint32_t id = order[k], c[2] = {t->lc[id], t->rc[id]};
+        ibox[id] = box_union(bin_box(ibox, leaf_boxes, c[0]), bin_box(ibox, leaf_boxes, c[1]));
+        t->area[id] = box_area(ibox[id]);
+        uint32_t cnt = 0;
+        for (int h = 0; h < 2; ++h) cnt += (c[h] & ORC_LEAF_BIT) ? 1u : t->count[c[h]];
+        t->count[id] = cnt;
+    }
+}
+
+/* Passes of treelet restructuring of a build: three with
+ * RTG_BUILD_PREFER_FAST_TRACE, none otherwise (the plain LBVH). */
+static int treelet_rounds(uint32_t flags) { return (flags & RTG_BUILD_PREFER_FAST_TRACE) ? 3 : 0; }
+
+/* Builds the wide tree over n primitive boxes, after `rounds` passes of
+ * treelet restructuring. With `keep_lc`/`keep_rc` (n - 1 entries each) the
+ * final binary tree is also returned (a test hook). */
+static int wide_build(const box3* prim_boxes, uint32_t n, uint32_t leaf_max, int rounds, wtree* out,
+                      uint32_t* keep_lc, uint32_t* keep_rc) {
     memset(out, 0, sizeof(*out));
     out->n = n;
     uint32_t cap = n >= 2 ? n - 1 : 1;
@@ -320,10 +430,35 @@ static int wide_build(const box3* prim_boxes, uint32_t n, uint32_t leaf_max, wtr
             node = parent_int[node];
         }
     }
+    {
+        for (int round = 0; round < rounds; ++round) {
+            memset(arrivals, 0, nint);
+            for (uint32_t k = 0; k < n; ++k) {
+                uint32_t node = parent_leaf[k];
+                while (node != 0xffffffffu) {
+                    if (arrivals[node]++ == 0) break;
+                    treelet_optimize(&t, ibox, leaf_boxes, node);
+                    node = parent_int[node];
+                }
+            }
+            /* parents of the restructured tree */
+            for (uint32_t i = 0; i < nint; ++i) {
+                uint32_t c[2] = {t.lc[i], t.rc[i]};
+                for (int s2 = 0; s2 < 2; ++s2) {
+                    if (c[s2] & ORC_LEAF_BIT) parent_leaf[c[s2] & ~ORC_LEAF_BIT] = i;
+                    else parent_int[c[s2]] = i;
+                }
+            }
+        }
+    }
     free(ibox);
     free(arrivals);
     free(parent_int);
     free(parent_leaf);
+    if (keep_lc && keep_rc) {
+        memcpy(keep_lc, t.lc, sizeof(uint32_t) * nint);
+        memcpy(keep_rc, t.rc, sizeof(uint32_t) * nint);
+    }
 
     /* top-down collapse, breadth first */
     uint32_t* root_bin = (uint32_t*)malloc(sizeof(uint32_t) * cap);
@@ -396,7 +531,8 @@ static v3 load_vertex(const rtg_geometry* g, uint32_t idx) {
     return V3(p[0], p[1], p[2]);
 }
 
-int orc_blas_build(const rtg_geometry* geoms, uint32_t geom_count, void** out_blob, uint64_t* out_size) {
+int orc_blas_build(const rtg_geometry* geoms, uint32_t geom_count, uint32_t flags, void** out_blob,
+                   uint64_t* out_size) {
     uint64_t total = 0;
     for (uint32_t g = 0; g < geom_count; ++g) {
         if (geoms[g].index_count % 3 != 0 || geoms[g].vertex_stride < 12 || geoms[g].vertex_stride % 4)
@@ -434,7 +570,7 @@ int orc_blas_build(const rtg_geometry* geoms, uint32_t geom_count, void** out_bl
         }
     }
     wtree w;
-    int st = wide_build(boxes, n, ORC_BLAS_LEAF_MAX, &w);
+    int st = wide_build(boxes, n, ORC_BLAS_LEAF_MAX, treelet_rounds(flags), &w, NULL, NULL);
     if (st) {
         free(boxes);
         free(tris);
@@ -489,12 +625,12 @@ static void fill_inst_record(orc_inst* rec, const rtg_instance* in, uint32_t ind
     rec->mask_flags = (in->custom_index_and_mask >> 24) | ((in->sbt_offset_and_flags >> 24) << 8);
 }
 
-int orc_tlas_build(const rtg_instance* inst, uint32_t count, void** out_blob, uint64_t* out_size) {
+int orc_tlas_build(const rtg_instance* inst, uint32_t count, uint32_t flags, void** out_blob, uint64_t* out_size) {
     if (count >= (1u << 27)) return RTG_ERROR_LIMIT;
     box3* boxes = (box3*)calloc(count ? count : 1, sizeof(box3));
     for (uint32_t i = 0; i < count; ++i) boxes[i] = instance_box(&inst[i]);
     wtree w;
-    int st = wide_build(boxes, count, ORC_TLAS_LEAF_MAX, &w);
+    int st = wide_build(boxes, count, ORC_TLAS_LEAF_MAX, treelet_rounds(flags), &w, NULL, NULL);
     free(boxes);
     if (st) {
         wtree_free(&w);
@@ -580,4 +716,21 @@ int orc_as_get_info(const void* blob, rtg_as_info* out) {
     }
     out->size_bytes = h->total_size;
     return RTG_OK;
+}
+
+int orc_debug_binary_tree(const float* boxes, uint32_t n, uint32_t flags, uint32_t* lc, uint32_t* rc,
+                          uint32_t* order) {
+    if (n < 2) return RTG_ERROR_INVALID_ARGUMENT;
+    box3* b = (box3*)malloc(sizeof(box3) * n);
+    for (uint32_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            b[i].lo[a] = boxes[6 * i + a];
+            b[i].hi[a] = boxes[6 * i + 3 + a];
+        }
+    wtree w;
+    int st = wide_build(b, n, 1u, treelet_rounds(flags), &w, lc, rc);
+    if (st == RTG_OK) memcpy(order, w.order, sizeof(uint32_t) * n);
+    wtree_free(&w);
+    free(b);
+    return st;
 }

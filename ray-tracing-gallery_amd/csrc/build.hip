@@ -472,6 +472,139 @@ __global__ __launch_bounds__(256) void k_bottom_up(const Box* boxes, const uint3
     }
 }
 
+// ---------------------------------------------------------------------------
+// Treelet restructuring (Karras & Aila 2013, HPG, "Fast Parallel
+// Construction of High-Quality Bounding Volume Hierarchies", section 4) of
+// the binary tree, for single-primitive leaves. The treelet of internal node
+// r: r's two children, then, while fewer than kTreeletLeaves, the treelet
+// leaf with the largest area that is an internal node replaced by its left
+// child in place and its right child appended. Its topology becomes the one
+// that minimises the summed surface area of its internal nodes (SAH cost with
+// fixed leaves), by dynamic programming over subsets of treelet leaves:
+// cost(S) = area(S) + min over partitions (P, S\P), P holding S's lowest
+// leaf, of cost(P) + cost(S\P); partitions tried in decreasing order of P,
+// the first minimum kept. New internal nodes reuse the treelet's numbers:
+// r for the whole set, then the expanded nodes in expansion order, handed
+// out as subsets are split from a stack (P's subtree first). A node is
+// processed after its whole subtree (arrival counters), so the result does
+// not depend on thread timing.
+// ---------------------------------------------------------------------------
+constexpr int kTreeletLeaves = 7;
+constexpr int kTreeletRounds = 3;  // passes with RTG_BUILD_PREFER_FAST_TRACE
+
+__device__ inline Box bin_box(const Box* boxes, const uint32_t* order, const Box* ibox, uint32_t c) {
+    return (c & kLeafBit) ? boxes[order[c & kFirstMask]] : ibox[c];
+}
+
+__device__ void treelet_optimize(uint32_t r, const Box* boxes, const uint32_t* order, uint32_t* left,
+                                 uint32_t* right, uint32_t* rcount, Box* ibox, float* area) {
+    uint32_t leaf[kTreeletLeaves], inner[kTreeletLeaves];
+    int nl = 2, ni = 0;
+    leaf[0] = left[r];
+    leaf[1] = right[r];
+    while (nl < kTreeletLeaves) {
+        int best = -1;
+        float best_area = 0.0f;
+        for (int k = 0; k < nl; ++k) {
+            const uint32_t c = leaf[k];
+            if (c & kLeafBit) continue;
+            const float a = area[c];
+            if (best < 0 || a > best_area) {
+                best = k;
+                best_area = a;
+            }
+        }
+        if (best < 0) break;
+        const uint32_t c = leaf[best];
+        inner[ni++] = c;
+        leaf[best] = left[c];
+        leaf[nl++] = right[c];
+    }
+    if (nl < 3) return;
+    Box lb[kTreeletLeaves];
+    for (int k = 0; k < nl; ++k) lb[k] = bin_box(boxes, order, ibox, leaf[k]);
+    const uint32_t full = (1u << nl) - 1u;
+    float cost[1 << kTreeletLeaves];
+    uint8_t split[1 << kTreeletLeaves];
+    for (uint32_t S = 1; S <= full; ++S) {
+        if ((S & (S - 1u)) == 0) {
+            cost[S] = 0.0f;
+            split[S] = 0;
+            continue;
+        }
+        Box b = box_empty();
+        for (int k = 0; k < nl; ++k)
+            if (S & (1u << k)) b = box_union(b, lb[k]);
+        const uint32_t low = S & (0u - S);
+        float best = 0.0f;
+        uint32_t bp = 0;
+        for (uint32_t P = (S - 1u) & S; P != 0; P = (P - 1u) & S) {
+            if (!(P & low)) continue;
+            const float c = cost[P] + cost[S ^ P];
+            if (bp == 0 || c < best) {
+                best = c;
+                bp = P;
+            }
+        }
+        cost[S] = box_area(b) + best;
+        split[S] = (uint8_t)bp;
+    }
+    uint32_t stk_set[kTreeletLeaves], stk_id[kTreeletLeaves], done[kTreeletLeaves];
+    int sp = 0, no = 0, next = 0;
+    stk_set[sp] = full;
+    stk_id[sp++] = r;
+    while (sp > 0) {
+        --sp;
+        const uint32_t S = stk_set[sp], id = stk_id[sp];
+        done[no++] = id;
+        const uint32_t part[2] = {split[S], S ^ split[S]};
+        uint32_t ref[2];
+        for (int h = 0; h < 2; ++h)
+            ref[h] = (part[h] & (part[h] - 1u)) ? inner[next++] : leaf[__builtin_ctz(part[h])];
+        left[id] = ref[0];
+        right[id] = ref[1];
+        for (int h = 1; h >= 0; --h)
+            if (part[h] & (part[h] - 1u)) {
+                stk_set[sp] = part[h];
+                stk_id[sp++] = ref[h];
+            }
+    }
+    for (int k = no - 1; k >= 0; --k) {
+        const uint32_t id = done[k], c0 = left[id], c1 = right[id];
+        const Box u = box_union(bin_box(boxes, order, ibox, c0), bin_box(boxes, order, ibox, c1));
+        ibox[id] = u;
+        area[id] = box_area(u);
+        rcount[id] = ((c0 & kLeafBit) ? 1u : rcount[c0]) + ((c1 & kLeafBit) ? 1u : rcount[c1]);
+    }
+}
+
+// One pass: every internal node restructured after its subtree.
+__global__ __launch_bounds__(256) void k_treelet(const Box* boxes, const uint32_t* order, int n, uint32_t* left,
+                                                 uint32_t* right, const uint32_t* parent, uint32_t* arrivals,
+                                                 uint32_t* rcount, Box* ibox, float* area) {
+    int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    uint32_t node = parent[(n - 1) + k];
+    while (node != 0xffffffffu) {
+        __threadfence();
+        uint32_t old = atomicAdd(&arrivals[node], 1u);
+        if (old == 0) return;
+        __threadfence();
+        treelet_optimize(node, boxes, order, left, right, rcount, ibox, area);
+        node = parent[node];
+    }
+}
+
+// Parents of the restructured tree (the root keeps none).
+__global__ __launch_bounds__(256) void k_parents(int n, const uint32_t* left, const uint32_t* right,
+                                                 uint32_t* parent) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n - 1) return;
+    const uint32_t lc = left[i], rc = right[i];
+    parent[(lc & kLeafBit) ? (n - 1) + (lc & kFirstMask) : lc] = (uint32_t)i;
+    parent[(rc & kLeafBit) ? (n - 1) + (rc & kFirstMask) : rc] = (uint32_t)i;
+}
+
 // The single node of a structure with at most leaf_max primitives: one leaf
 // of all of them in input order (or none).
 __global__ void k_wide_single(const Box* boxes, uint32_t n, Node* nodes, uint32_t* order) {
@@ -778,7 +911,7 @@ static LbvhBufs carve_lbvh(void* base, uint32_t n, size_t rec_bytes, uint32_t ng
 // the wide tree in b.nodes. On return the leaf order is in b.va and
 // level_off holds the level offsets (level_off.back() = node count). Reads
 // each level's size back from the device (synchronises `s`).
-static int wide_hierarchy(LbvhBufs& b, uint32_t n, uint32_t leaf_max, hipStream_t s,
+static int wide_hierarchy(LbvhBufs& b, uint32_t n, uint32_t leaf_max, int treelet_rounds, hipStream_t s,
                           std::vector<uint32_t>& level_off) {
     level_off.assign(1, 0u);
     if (n <= leaf_max || n <= 1) {
@@ -805,6 +938,15 @@ static int wide_hierarchy(LbvhBufs& b, uint32_t n, uint32_t leaf_max, hipStream_
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.arrivals, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_bottom_up, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.boxes, b.va, (int)n, b.left, b.right,
                        b.parent, b.arrivals, b.ibox, b.area);
+    // the treelet passes need single-primitive leaves: a restructured
+    // subtree no longer covers a contiguous range of sorted primitives
+    for (int round = 0; round < treelet_rounds && leaf_max == 1; ++round) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.arrivals, (uint64_t)n, 0u);
+        hipLaunchKernelGGL(k_treelet, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.boxes, b.va, (int)n, b.left,
+                           b.right, b.parent, b.arrivals, b.rcount, b.ibox, b.area);
+        hipLaunchKernelGGL(k_parents, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, (int)n, b.left, b.right,
+                           b.parent);
+    }
     // top-down collapse, one breadth-first level per step
     const uint32_t cap = node_cap(n);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, b.wroot, 1ull, 0u);
@@ -941,7 +1083,8 @@ extern "C" int rtg_blas_build(rtg_ctx* ctx, const rtg_geometry* geoms, uint32_t 
         hipLaunchKernelGGL(k_tri_setup, dim3(blocks_for(n, 256)), dim3(256), 0, s, d_geoms, n_geom, n,
                            reinterpret_cast<TriRec*>(b.recs), b.boxes, b.cb, ctx->d_status);
     std::vector<uint32_t> level_off;
-    int st = wide_hierarchy(b, n, kBlasLeafMax, s, level_off);
+    int st = wide_hierarchy(b, n, kBlasLeafMax, (flags & RTG_BUILD_PREFER_FAST_TRACE) ? kTreeletRounds : 0, s,
+                            level_off);
     if (st != RTG_OK) {
         hipStreamSynchronize(s);  // pending copies read host memory of this frame
         return st;
@@ -977,7 +1120,6 @@ extern "C" int rtg_blas_build(rtg_ctx* ctx, const rtg_geometry* geoms, uint32_t 
     ctx->structures[addr] = AsRecord{kKindBlas, n, L.node_count, L.total};
     st = refresh_registry(ctx);
     if (st != RTG_OK) return st;
-    (void)flags;
     *out = addr;
     return RTG_OK;
 }
@@ -998,7 +1140,8 @@ extern "C" int rtg_tlas_build(rtg_ctx* ctx, const rtg_instance* instances, uint3
         hipLaunchKernelGGL(k_inst_setup, dim3(blocks_for(n, 256)), dim3(256), 0, s, instances, n, ctx->d_registry,
                            ctx->registry_count, reinterpret_cast<InstRec*>(b.recs), b.boxes, b.cb, ctx->d_status);
     std::vector<uint32_t> level_off;
-    int st = wide_hierarchy(b, n, kTlasLeafMax, s, level_off);
+    int st = wide_hierarchy(b, n, kTlasLeafMax, (flags & RTG_BUILD_PREFER_FAST_TRACE) ? kTreeletRounds : 0, s,
+                            level_off);
     if (st != RTG_OK) {
         hipStreamSynchronize(s);
         return st;
@@ -1040,7 +1183,6 @@ extern "C" int rtg_tlas_build(rtg_ctx* ctx, const rtg_instance* instances, uint3
     }
     uint64_t addr = reinterpret_cast<uint64_t>(blob);
     ctx->structures[addr] = AsRecord{kKindTlas, n, L.node_count, L.total};
-    (void)flags;
     *out = addr;
     return RTG_OK;
 }

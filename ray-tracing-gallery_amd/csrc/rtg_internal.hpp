@@ -71,6 +71,7 @@ enum StatusBits : uint32_t {
     kStatusBadIndex = RTG_STATUS_BAD_INDEX,            // BLAS build: index > max_vertex
     kStatusBadBlas = RTG_STATUS_UNKNOWN_BLAS,          // TLAS build/refit: reference is not a live BLAS
     kStatusStackOverflow = RTG_STATUS_STACK_OVERFLOW,  // trace: traversal stack exhausted
+    kStatusQueueRange = RTG_STATUS_QUEUE_RANGE,        // trace: queue size or entry out of range
 };
 
 struct TimedLaunch {

@@ -106,7 +106,15 @@ struct Frame {
     int refill_occlusion;
     uint32_t feed_chunk;      // queue entries a wave takes per atomic
     uint32_t side;            // 1 on the bounce stream (counts its shadow rays apart)
+    // capacities of the path and shadow queues: sizes read from the device
+    // are clamped to them and entries checked, so that a frame state driven
+    // by two traces at once drops rays (kStatusQueueRange) instead of
+    // reading or writing outside its buffers
+    uint32_t path_cap, shadow_cap;
+    uint32_t* status;
 };
+
+__device__ inline void queue_range_error(const Frame& F) { atomicOr(F.status, (uint32_t)kStatusQueueRange); }
 
 __device__ inline uint32_t find_job(const JobDev* jobs, uint32_t njobs, uint64_t p, bool by_pixel) {
     uint32_t lo = 0, hi = njobs;
@@ -291,7 +299,7 @@ template <bool COUNT, bool SIDE>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int level, const uint32_t* count_ptr,
                                                                  uint32_t fixed_count, uint32_t* status) {
     __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
-    const uint32_t n = count_ptr ? *count_ptr : fixed_count;
+    const uint32_t n = min(count_ptr ? *count_ptr : fixed_count, F.path_cap);
     const Queue Q = F.q[level & 1];
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
     Stack st;
@@ -314,6 +322,10 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int le
             if (id != ~0u) {
                 slot = id;
                 path = Q.path[id];
+                if (path >= F.paths) {
+                    queue_range_error(F);
+                    path = 0;  // traced as path 0's ray; the shade stage drops it
+                }
                 J = &F.jobs[find_job(F.jobs, F.njobs, path, false)];
                 T = tlas_view(J->tlas);
                 float4 a = Q.o[id], b = Q.d[id];
@@ -344,7 +356,7 @@ template <bool COUNT, bool SIDE>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_closest_rounds(Frame F, int level, const uint32_t* count_ptr,
                                                                         uint32_t fixed_count, uint32_t* status) {
     __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
-    const uint32_t n = count_ptr ? *count_ptr : fixed_count;
+    const uint32_t n = min(count_ptr ? *count_ptr : fixed_count, F.path_cap);
     const Queue Q = F.q[level & 1];
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
     Stack st;
@@ -358,6 +370,10 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest_rounds(Frame F,
         uint32_t i = r * stride + gtid;
         if (i < n) {
             uint32_t p = Q.path[i];
+            if (p >= F.paths) {
+                queue_range_error(F);
+                p = 0;  // traced as path 0's ray; the shade stage drops it
+            }
             const JobDev& J = F.jobs[find_job(F.jobs, F.njobs, p, false)];
             const TlasView T = tlas_view(J.tlas);
             float4 a = Q.o[i], b = Q.d[i];
@@ -453,7 +469,7 @@ __device__ inline void brdf_parts(V3 n, V3 view, V3 l, float pr, V3 base, float 
 // Hit and miss stages of trace level b.
 template <bool COUNT>
 __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_t* count_ptr, uint32_t fixed_count) {
-    const uint32_t n = count_ptr ? *count_ptr : fixed_count;
+    const uint32_t n = min(count_ptr ? *count_ptr : fixed_count, F.path_cap);
     const Queue Q = F.q[level & 1];
     const Queue NQ = F.q[(level + 1) & 1];
     uint32_t* next_count = &F.counts[level + 1];
@@ -472,6 +488,12 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
         const JobDev* J = nullptr;
         if (active) {
             p = Q.path[i];
+            if (p >= F.paths) {
+                queue_range_error(F);
+                active = false;
+            }
+        }
+        if (active) {
             J = &F.jobs[find_job(F.jobs, F.njobs, p, false)];
             uint64_t local = p - J->sample_base;
             uint32_t s = (uint32_t)(local % J->spp);
@@ -589,6 +611,14 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
         // compaction: one atomic per wave for each queue
         uint32_t sbase = wave_alloc(want_shadow, &F.counts[4]);
         uint32_t nbase = wave_alloc(want_next ? 1u : 0u, next_count);
+        if (want_next && nbase >= F.path_cap) {
+            queue_range_error(F);
+            want_next = false;
+        }
+        if (want_shadow && sbase + want_shadow > F.shadow_cap) {
+            queue_range_error(F);
+            want_shadow = 0;
+        }
         if (want_next) {
             NQ.o[nbase] = make_float4(next_o.x, next_o.y, next_o.z, 0.01f);
             NQ.d[nbase] = make_float4(next_d.x, next_d.y, next_d.z, 10000.0f);
@@ -645,8 +675,8 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
 // rays of the first trace level (counts[kShadowFirst] of them, the queue's
 // size when the first shade finished) and the rays appended after them.
 __device__ inline void shadow_part(const Frame& F, int part, uint32_t& lo, uint32_t& hi, uint32_t*& feed) {
-    lo = part == kPartRest ? F.counts[kShadowFirst] : 0u;
-    hi = part == kPartFirst ? F.counts[kShadowFirst] : F.counts[4];
+    hi = min(part == kPartFirst ? F.counts[kShadowFirst] : F.counts[4], F.shadow_cap);
+    lo = min(part == kPartRest ? F.counts[kShadowFirst] : 0u, hi);
     feed = &F.counts[part == kPartRest ? kFeedShadowRest : kFeedShadow];
 }
 
@@ -678,6 +708,10 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion(Frame F, uint
             if (id != ~0u) {
                 id += lo;
                 path = F.sq.path[id];
+                if (path >= F.paths) {
+                    queue_range_error(F);
+                    path = 0;
+                }
                 J = &F.jobs[find_job(F.jobs, F.njobs, path, false)];
                 T = tlas_view(J->tlas);
                 float4 a = F.sq.o[id], b = F.sq.d[id];
@@ -721,6 +755,10 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion_rounds(Frame 
         uint32_t i = lo + r * stride + gtid;
         if (i < hi) {
             uint32_t p = F.sq.path[i];
+            if (p >= F.paths) {
+                queue_range_error(F);
+                p = 0;
+            }
             const JobDev& J = F.jobs[find_job(F.jobs, F.njobs, p, false)];
             const TlasView T = tlas_view(J.tlas);
             float4 a = F.sq.o[i], b = F.sq.d[i];
@@ -1080,6 +1118,9 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
     F.refill_occlusion = tune.refill_occlusion;
     F.feed_chunk = (uint32_t)tune.feed_chunk;
     F.side = 0;
+    F.path_cap = (uint32_t)std::min<uint64_t>(a->paths_cap, 0xffffffffu);
+    F.shadow_cap = (uint32_t)std::min<uint64_t>(a->shadow_cap, 0xffffffffu);
+    F.status = ctx->d_status;
     const unsigned G = (unsigned)trace_grid(ctx, tune);
     const unsigned Gsmall = (unsigned)std::min<uint64_t>((P.paths + 255) / 256, (uint64_t)ctx->cu_count * 16);
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, a->d_counts, ctx->d_counters, cnt);

@@ -46,6 +46,7 @@ RTG_TRACE_OUTPUT_BGRA8 = 0x2
 RTG_STATUS_BAD_INDEX = 0x1
 RTG_STATUS_UNKNOWN_BLAS = 0x2
 RTG_STATUS_STACK_OVERFLOW = 0x4
+RTG_STATUS_QUEUE_RANGE = 0x8
 
 RTG_KERNEL_NAMES = ["raygen", "trace_closest", "shade", "trace_occlusion", "resolve", "refit", "batch_graph",
                     "bounce_side"]

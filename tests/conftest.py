@@ -37,9 +37,10 @@ def ctx():
 
 @pytest.fixture(autouse=True)
 def no_stack_overflow(request):
-    """Every GPU test ends with no traversal-stack overflow on its context."""
+    """Every GPU test ends with no traversal-stack overflow and no ray-queue
+    range error on its context."""
     yield
     if "ctx" in request.fixturenames and request.node.get_closest_marker("gpu"):
         from rtg import abi
         c = request.getfixturevalue("ctx")
-        assert not (c.status() & abi.RTG_STATUS_STACK_OVERFLOW)
+        assert not (c.status() & (abi.RTG_STATUS_STACK_OVERFLOW | abi.RTG_STATUS_QUEUE_RANGE))

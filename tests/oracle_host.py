@@ -26,8 +26,9 @@ def lib():
             raise RuntimeError(f"oracle not built: {ORC_PATH} (run make -C oracle)")
         L = C.CDLL(ORC_PATH)
         vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
-        L.orc_blas_build.argtypes = [vp, u32, C.POINTER(vp), C.POINTER(u64)]
-        L.orc_tlas_build.argtypes = [vp, u32, C.POINTER(vp), C.POINTER(u64)]
+        L.orc_blas_build.argtypes = [vp, u32, u32, C.POINTER(vp), C.POINTER(u64)]
+        L.orc_tlas_build.argtypes = [vp, u32, u32, C.POINTER(vp), C.POINTER(u64)]
+        L.orc_debug_binary_tree.argtypes = [vp, u32, u32, vp, vp, vp]
         L.orc_tlas_refit.argtypes = [vp, vp, u32]
         L.orc_free.argtypes = [vp]
         L.orc_as_get_info.argtypes = [vp, C.POINTER(abi.AsInfo)]
@@ -101,12 +102,17 @@ def geometries_for(mesh, keep):
     return arr
 
 
-def build_blas(mesh, keep=None):
+# the flags rtg/device.py builds with (the reference's PREFER_FAST_TRACE)
+BLAS_FLAGS = abi.RTG_BUILD_PREFER_FAST_TRACE
+TLAS_FLAGS = abi.RTG_BUILD_PREFER_FAST_TRACE | abi.RTG_BUILD_ALLOW_UPDATE
+
+
+def build_blas(mesh, keep=None, flags=BLAS_FLAGS):
     keep = [] if keep is None else keep
     arr = geometries_for(mesh, keep)
     out = C.c_void_p()
     size = C.c_uint64()
-    st = lib().orc_blas_build(C.cast(arr, C.c_void_p), len(mesh.geometries), C.byref(out), C.byref(size))
+    st = lib().orc_blas_build(C.cast(arr, C.c_void_p), len(mesh.geometries), flags, C.byref(out), C.byref(size))
     if st != 0:
         raise RuntimeError(f"orc_blas_build failed: {st}")
     return HostBlob(out.value, size.value)
@@ -116,11 +122,11 @@ class HostScene:
     """A Scene's buffers, descriptor tables and acceleration structures in host
     memory for the oracle."""
 
-    def __init__(self, scene):
+    def __init__(self, scene, blas_flags=BLAS_FLAGS, tlas_flags=TLAS_FLAGS):
         L = lib()
         self.scene = scene
         self.keep = []
-        self.blas = [build_blas(m, self.keep) for m in scene.meshes]
+        self.blas = [build_blas(m, self.keep, blas_flags) for m in scene.meshes]
         # textures
         self.tex_table = np.zeros(max(1, len(scene.textures)), dtype=abi.TEXTURE_DTYPE)
         for k, t in enumerate(scene.textures):
@@ -149,7 +155,7 @@ class HostScene:
         self.instances = scene.instance_array([b.ptr for b in self.blas])
         out = C.c_void_p()
         size = C.c_uint64()
-        st = L.orc_tlas_build(_ptr(self.instances), len(self.instances), C.byref(out), C.byref(size))
+        st = L.orc_tlas_build(_ptr(self.instances), len(self.instances), tlas_flags, C.byref(out), C.byref(size))
         if st != 0:
             raise RuntimeError(f"orc_tlas_build failed: {st}")
         self.tlas = HostBlob(out.value, size.value)
@@ -210,3 +216,17 @@ class HostScene:
         if st != 0:
             raise RuntimeError(f"orc_trace_rays failed: {st}")
         return (hits if mode == abi.RTG_QUERY_CLOSEST else occ), cnt
+
+
+def binary_tree(boxes, flags=BLAS_FLAGS):
+    """The binary tree (lc, rc, order) the oracle's build over `boxes` ((n, 6)
+    float32: lo xyz, hi xyz; n >= 2) collapses into wide nodes."""
+    b = np.ascontiguousarray(boxes, dtype=np.float32)
+    n = len(b)
+    lc = np.zeros(n - 1, np.uint32)
+    rc = np.zeros(n - 1, np.uint32)
+    order = np.zeros(n, np.uint32)
+    st = lib().orc_debug_binary_tree(_ptr(b), n, flags, _ptr(lc), _ptr(rc), _ptr(order))
+    if st != 0:
+        raise RuntimeError(f"orc_debug_binary_tree failed: {st}")
+    return lc, rc, order

@@ -140,7 +140,7 @@ def test_blas_is_the_collapsed_radix_tree_of_sorted_morton_keys(kind):
         base = scenes.torus(6, 4)
         ix = np.concatenate([base.geometries[0][0]] * 5)
         mesh = scenes.Mesh("dups", base.positions, base.normals, base.uvs, [(ix, True)])
-    blob = Blob(oh.build_blas(mesh).bytes())
+    blob = Blob(oh.build_blas(mesh, flags=0).bytes())  # the plain LBVH
     boxes = tri_boxes(mesh)
     n = len(boxes)
     order = [int(t["prim"]) for t in blob.prims]  # single geometry: prim = original index
@@ -207,7 +207,7 @@ def test_tiny_blas(n):
     the sorted positions at their highest differing bit."""
     mesh = scenes.Mesh("tiny", [[0, 0, 0], [1, 0, 0], [0, 1, 0]], [[0, 0, 1]] * 3, [[0, 0]] * 3,
                        [(np.array([[0, 1, 2]] * n).reshape(-1, 3), True)])
-    blob = Blob(oh.build_blas(mesh).bytes())
+    blob = Blob(oh.build_blas(mesh, flags=0).bytes())  # the plain LBVH
     assert [int(t["prim"]) for t in blob.prims] == list(range(n))  # equal keys keep the input order
     if n <= BLAS_LEAF_MAX or n <= 1:
         # one node, one leaf of all primitives (or none)
@@ -288,3 +288,119 @@ def test_tlas_refit_equals_fresh_build_bounds():
         w = np.eye(4)
         w[:3] = r["w2o"].reshape(3, 4)
         np.testing.assert_allclose(w @ m, np.eye(4), atol=2e-5)
+
+
+# --- treelet restructuring (RTG_BUILD_PREFER_FAST_TRACE) --------------------
+
+FAST = oh.BLAS_FLAGS
+
+
+def _tree_leaves(lc, rc, node, out):
+    for c in (int(lc[node]), int(rc[node])):
+        if c & LEAF:
+            out.append(c & ~LEAF)
+        else:
+            _tree_leaves(lc, rc, c, out)
+    return out
+
+
+def _internal_area_sum(lc, rc, boxes_sorted):
+    """Summed surface area (float64, of the float32 boxes) of all internal
+    nodes of a binary tree, and each node's leaf list."""
+    total = 0.0
+    for i in range(len(lc)):
+        leaves = _tree_leaves(lc, rc, i, [])
+        b = boxes_sorted[leaves]
+        e = b[:, 3:].max(0).astype(np.float64) - b[:, :3].min(0).astype(np.float64)
+        total += e[0] * e[1] + e[1] * e[2] + e[2] * e[0]
+    return total
+
+
+def _all_trees(items):
+    """Every full binary tree over the leaf set `items` (nested pairs)."""
+    if len(items) == 1:
+        yield items[0]
+        return
+    first, rest = items[0], items[1:]
+    for mask in range(1 << len(rest)):
+        if mask == (1 << len(rest)) - 1:
+            continue  # the other side would be empty
+        left = [first] + [x for k, x in enumerate(rest) if mask >> k & 1]
+        right = [x for k, x in enumerate(rest) if not mask >> k & 1]
+        for tl in _all_trees(left):
+            for tr in _all_trees(right):
+                yield (tl, tr)
+
+
+def _tree_cost(t, boxes):
+    """Summed float64 area of the internal nodes of a nested-pair tree."""
+    def walk(t):
+        if not isinstance(t, tuple):
+            return [t], 0.0
+        la, ca = walk(t[0])
+        lb, cb = walk(t[1])
+        leaves = la + lb
+        b = boxes[leaves]
+        e = b[:, 3:].max(0).astype(np.float64) - b[:, :3].min(0).astype(np.float64)
+        return leaves, ca + cb + e[0] * e[1] + e[1] * e[2] + e[2] * e[0]
+    return walk(t)[1]
+
+
+def _random_boxes(n, seed):
+    rng = np.random.default_rng(seed)
+    lo = rng.uniform(-4, 4, (n, 3)).astype(np.float32)
+    size = rng.lognormal(-1.0, 1.0, (n, 3)).astype(np.float32)
+    return np.concatenate([lo, lo + size], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("n,seed", [(3, 1), (4, 2), (5, 3), (6, 4), (7, 5), (7, 6)])
+def test_treelet_tree_is_sah_optimal_up_to_seven_leaves(n, seed):
+    """With at most seven primitives the root's treelet holds them all, so the
+    restructured tree must have the least summed internal-node area of all
+    (2n-3)!! binary trees over them (found here by enumeration)."""
+    boxes = _random_boxes(n, seed)
+    lc, rc, order = oh.binary_tree(boxes, FAST)
+    sb = boxes[order]
+    got = _internal_area_sum(lc, rc, sb)
+    best = min(_tree_cost(t, sb) for t in _all_trees(list(range(n))))
+    assert got <= best * (1 + 1e-6)
+    plain = _internal_area_sum(*oh.binary_tree(boxes, 0)[:2], sb)
+    assert got <= plain * (1 + 1e-6)
+
+
+@pytest.mark.parametrize("kind", ["soup", "torus", "grid"])
+def test_treelet_tree_is_valid_and_never_costlier(kind):
+    if kind == "soup":
+        boxes = tri_boxes(scenes.soup(700, 0.05, rng=scenes.Rng(3))).astype(np.float32)
+    elif kind == "torus":
+        boxes = tri_boxes(scenes.torus(24, 12)).astype(np.float32)
+    else:  # equal boxes on a grid: ties everywhere
+        g = np.stack(np.meshgrid(*[np.arange(6, dtype=np.float32)] * 3), -1).reshape(-1, 3)
+        boxes = np.concatenate([g, g + 0.5], 1).astype(np.float32)
+    n = len(boxes)
+    lc, rc, order = oh.binary_tree(boxes, FAST)
+    assert sorted(order.tolist()) == list(range(n))
+    # every internal node but the root has one parent; every leaf appears once
+    kids = [int(c) for c in np.concatenate([lc, rc])]
+    internal = sorted(c for c in kids if not c & LEAF)
+    assert internal == list(range(1, n - 1))
+    assert sorted(c & ~LEAF for c in kids if c & LEAF) == list(range(n))
+    sb = boxes[order]
+    plain_lc, plain_rc, plain_order = oh.binary_tree(boxes, 0)
+    assert np.array_equal(order, plain_order)  # the same sorted primitives
+    assert _internal_area_sum(lc, rc, sb) <= _internal_area_sum(plain_lc, plain_rc, sb) * (1 + 1e-6)
+
+
+def test_treelet_structures_give_the_same_frame():
+    """Restructured BLASes and TLAS change which nodes are visited (fewer),
+    never what is hit: the frame, hit records included, equals the plain
+    LBVH's bit for bit."""
+    s = scenes.large_scene(width=64, height=40, n_objects=40, n_fences=6, n_mirrors=4, n_soup_meshes=2,
+                           soup_tris=300)
+    fast = oh.HostScene(s).trace_image()
+    plain = oh.HostScene(s, blas_flags=0, tlas_flags=oh.abi.RTG_BUILD_ALLOW_UPDATE).trace_image()
+    assert np.array_equal(fast[0], plain[0])
+    assert np.array_equal(fast[1], plain[1])
+    assert np.array_equal(fast[2].view(np.uint32), plain[2].view(np.uint32))
+    assert fast[3].node_visits < plain[3].node_visits
+    assert fast[3].occ_node_visits < plain[3].occ_node_visits
