@@ -175,7 +175,7 @@ typedef struct rtg_geometry {
 
 /* Build flags (vk::BuildAccelerationStructureFlagsKHR). PREFER_FAST_TRACE is
  * what the reference passes (util_structs.rs:194, util_functions.rs:480-483):
- * it adds three passes of treelet restructuring (SAH-optimal topology of
+ * it adds five passes of treelet restructuring (SAH-optimal topology of
  * 7-leaf treelets) to the LBVH, about a quarter fewer node visits per ray on
  * instanced scenes; without it the build is the plain LBVH. ALLOW_UPDATE is
  * accepted for TLASes, which can always be refitted. */

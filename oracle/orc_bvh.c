@@ -335,9 +335,9 @@ static void treelet_optimize(bintree* t, box3* ibox, const box3* leaf_boxes, uin
     }
 }
 
-/* Passes of treelet restructuring of a build: three with
+/* Passes of treelet restructuring of a build: five with
  * RTG_BUILD_PREFER_FAST_TRACE, none otherwise (the plain LBVH). */
-static int treelet_rounds(uint32_t flags) { return (flags & RTG_BUILD_PREFER_FAST_TRACE) ? 3 : 0; }
+static int treelet_rounds(uint32_t flags) { return (flags & RTG_BUILD_PREFER_FAST_TRACE) ? 5 : 0; }
 
 /* Builds the wide tree over n primitive boxes, after `rounds` passes of
  * treelet restructuring. With `keep_lc`/`keep_rc` (n - 1 entries each) the

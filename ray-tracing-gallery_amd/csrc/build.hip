@@ -490,7 +490,7 @@ __global__ __launch_bounds__(256) void k_bottom_up(const Box* boxes, const uint3
 // not depend on thread timing.
 // ---------------------------------------------------------------------------
 constexpr int kTreeletLeaves = 7;
-constexpr int kTreeletRounds = 3;  // passes with RTG_BUILD_PREFER_FAST_TRACE
+constexpr int kTreeletRounds = 5;  // passes with RTG_BUILD_PREFER_FAST_TRACE
 
 __device__ inline Box bin_box(const Box* boxes, const uint32_t* order, const Box* ibox, uint32_t c) {
     return (c & kLeafBit) ? boxes[order[c & kFirstMask]] : ibox[c];
