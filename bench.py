@@ -54,6 +54,17 @@ def algorithmic_bytes(c, kind):
             (c["occ_instance_visits"] - c["side_occ_instance_visits"]) * 112)
 
 
+def measured_l2_hit_rate(kernel):
+    """L2 (TCC) hit rate of `kernel`'s launches from the committed rocprofv3
+    summary of this benchmark (TCC_HIT / (TCC_HIT + TCC_MISS)), or None."""
+    path = os.path.join(REPO, "profiles", "hbm_traffic_trace_kernels.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("l2_hit_rate", {}).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by
     the copy-kernel calibration) from the committed rocprofv3 summary of this
@@ -318,7 +329,8 @@ def incoherent_benchmark(ctx, n_rays=16 << 20, sample=65536):
         out[name] = {"ms": round(ms, 3), "mrays_s": round(n_rays / (ms / 1e3) / 1e6, 1),
                      "nodes_per_ray": round(nodes / n_rays, 2), "tris_per_ray": round(trisv / n_rays, 2),
                      "algorithmic_gb_s": round(nb / (ms / 1e3) / 1e9, 1),
-                     "measured_traffic_gb_s": round(traffic / (ms / 1e3) / 1e9, 1) if traffic else None}
+                     "measured_traffic_gb_s": round(traffic / (ms / 1e3) / 1e9, 1) if traffic else None,
+                     "l2_hit_rate": measured_l2_hit_rate(f"k_trace_rays_{name}")}
     torch.cuda.synchronize()
     # bit-exact check of a ray sample against the oracle (same BVH, same order)
     sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -443,6 +455,7 @@ def main():
     per_launch_bytes = bytes_total / max(1, launches)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(f"k_trace_{dom}"),
+                "l2_hit_rate": measured_l2_hit_rate(f"k_trace_{dom}"),
                 "kernel": f"k_trace_{'closest' if dom == 'closest' else 'occlusion'}",
                 "avg_launch_ms": round(ms / max(1, launches), 4),
                 "algorithmic_bytes_per_launch": int(per_launch_bytes)}
