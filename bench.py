@@ -99,9 +99,10 @@ def cpu_baseline(scene, threads, min_seconds=10.0):
 
 # (requests per batch, batches in flight, samples per batch): plain = one
 # request per launch, one at a time; served = continuous batching with up to
-# 3 batches in flight of at most 2 M samples each (the setting with the best
-# p99 in a sweep on MI355X; larger batches delay small requests)
-SERVE_MODES = {"plain": (1, 1, 0), "served": (32, 3, 2 << 20)}
+# 4 batches in flight of at most 1 M samples each (the setting with the best
+# p99 in a sweep on MI355X with 8 hardware queues; larger batches delay small
+# requests)
+SERVE_MODES = {"plain": (1, 1, 0), "served": (32, 4, 1 << 20)}
 
 
 def serving_benchmark(ctx, n_scenes=32, n_requests=160, seed=0x5EED0004, modes=None):
@@ -109,7 +110,7 @@ def serving_benchmark(ctx, n_scenes=32, n_requests=160, seed=0x5EED0004, modes=N
     256x256, 1280x720 or 1920x1080 with 1-8 samples per pixel, each with a
     random instance-transform update (a refit of a private TLAS copy).
     Plain = one request per launch, one at a time; served = continuous
-    batching (up to 32 requests and 2 M samples per batch, 3 batches in
+    batching (up to 32 requests and 1 M samples per batch, 4 batches in
     flight). Returns
     saturation throughput (all requests at t=0) and p50/p99 latency at a
     Poisson arrival rate of 0.6x the plain path's saturation throughput."""
