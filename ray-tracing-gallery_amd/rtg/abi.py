@@ -20,6 +20,7 @@ RTG_INSTANCE_FORCE_NO_OPAQUE = 0x8
 RTG_HIT_TEXTURED = 0
 RTG_HIT_MIRROR = 1
 RTG_HIT_PORTAL = 2
+RTG_HIT_GROUP_COUNT = 3
 
 RTG_FORMAT_RGBA8_UNORM = 0
 RTG_FORMAT_RGBA8_SRGB = 1
