@@ -274,8 +274,10 @@ static int traverse(qctx* q, v3 o, v3 d, float tmin, float tmax, int occlusion, 
                 const orc_node* nd = &nodes[cur];
                 kc[0]++;
                 /* the overlapped slots in ascending (entry distance, slot)
-                   order: the first is visited next, the others are pushed
-                   so that the second is on top */
+                   order for closest-hit queries, in slot order for
+                   occlusion queries (any hit ends them, so no order is
+                   needed for the result): the first is visited next, the
+                   others are pushed so that the second is on top */
                 uint32_t ref[4];
                 float tn[4];
                 int nh = 0;
@@ -283,7 +285,7 @@ static int traverse(qctx* q, v3 o, v3 d, float tmin, float tmax, int occlusion, 
                     float t = 0.0f;
                     if (nd->child[s] == ORC_EMPTY || !box_test(nd, s, cr, tmin, tcur, &t)) continue;
                     int k = nh++;
-                    while (k > 0 && t < tn[k - 1]) {
+                    while (!occlusion && k > 0 && t < tn[k - 1]) {
                         ref[k] = ref[k - 1];
                         tn[k] = tn[k - 1];
                         --k;

@@ -551,8 +551,10 @@ constexpr int kRayOverflow = 3;
 // Two-level traversal as a resumable state machine: start() sets up a ray,
 // each step() does one visit (an internal node, an instance entry or a leaf
 // with the pops that follow it). The order of visits is fixed: depth first,
-// nearer child first (ties to child 0), popped entries skipped once their
-// entry distance exceeds the closest hit.
+// nearer child first (ties to child 0) for closest-hit queries, children in
+// slot order for occlusion queries (any accepted hit ends them, so the order
+// only changes how soon), popped entries skipped once their entry distance
+// exceeds the closest hit.
 template <bool OCCLUSION, bool COUNT>
 struct Traverser {
     RayCtx world, obj;
@@ -624,11 +626,19 @@ struct Traverser {
                 return kRayActive;
             }
             // rank of each overlapped slot in ascending (entry distance, slot)
-            // order; the rank-0 slot is visited next, ranks nh-1 .. 1 are
-            // pushed. A rank held by none of slots 0-2 is slot 3's.
-            const int r0 = (h1 && t1 < t0) + (h2 && t2 < t0) + (h3 && t3 < t0);
-            const int r1 = (h0 && t0 <= t1) + (h2 && t2 < t1) + (h3 && t3 < t1);
-            const int r2 = (h0 && t0 <= t2) + (h1 && t1 <= t2) + (h3 && t3 < t2);
+            // order (slot order for occlusion); the rank-0 slot is visited
+            // next, ranks nh-1 .. 1 are pushed. A rank held by none of slots
+            // 0-2 is slot 3's.
+            int r0, r1, r2;
+            if (OCCLUSION) {
+                r0 = 0;
+                r1 = (int)h0;
+                r2 = (int)h0 + (int)h1;
+            } else {
+                r0 = (h1 && t1 < t0) + (h2 && t2 < t0) + (h3 && t3 < t0);
+                r1 = (h0 && t0 <= t1) + (h2 && t2 < t1) + (h3 && t3 < t1);
+                r2 = (h0 && t0 <= t2) + (h1 && t1 <= t2) + (h3 && t3 < t2);
+            }
             if (st.sp + nh - 1 > kStackMax) return kRayOverflow;
 #pragma unroll
             for (int q = 3; q >= 1; --q) {
