@@ -625,21 +625,22 @@ struct Traverser {
                 if (!pop(T, st)) return done();
                 return kRayActive;
             }
-            // rank of each overlapped slot in ascending (entry distance, slot)
-            // order (slot order for occlusion); the rank-0 slot is visited
-            // next, ranks nh-1 .. 1 are pushed. A rank held by none of slots
-            // 0-2 is slot 3's.
-            int r0, r1, r2;
-            if (OCCLUSION) {
-                r0 = 0;
-                r1 = (int)h0;
-                r2 = (int)h0 + (int)h1;
-            } else {
-                r0 = (h1 && t1 < t0) + (h2 && t2 < t0) + (h3 && t3 < t0);
-                r1 = (h0 && t0 <= t1) + (h2 && t2 < t1) + (h3 && t3 < t1);
-                r2 = (h0 && t0 <= t2) + (h1 && t1 <= t2) + (h3 && t3 < t2);
-            }
             if (st.sp + nh - 1 > kStackMax) return kRayOverflow;
+            if (OCCLUSION) {
+                // slot order: the lowest overlapped slot is visited next, the
+                // others are pushed highest first
+                if (h3 && (h0 || h1 || h2)) st.push(C.w, t3);
+                if (h2 && (h0 || h1)) st.push(C.z, t2);
+                if (h1 && h0) st.push(C.y, t1);
+                cur = h0 ? C.x : h1 ? C.y : h2 ? C.z : C.w;
+                return kRayActive;
+            }
+            // rank of each overlapped slot in ascending (entry distance, slot)
+            // order; the rank-0 slot is visited next, ranks nh-1 .. 1 are
+            // pushed. A rank held by none of slots 0-2 is slot 3's.
+            const int r0 = (h1 && t1 < t0) + (h2 && t2 < t0) + (h3 && t3 < t0);
+            const int r1 = (h0 && t0 <= t1) + (h2 && t2 < t1) + (h3 && t3 < t1);
+            const int r2 = (h0 && t0 <= t2) + (h1 && t1 <= t2) + (h3 && t3 < t2);
 #pragma unroll
             for (int q = 3; q >= 1; --q) {
                 if (q < nh) {
