@@ -193,14 +193,15 @@ void orc_watertight(const float o[3], const float d[3], const float v0[3], const
 /* Robust slab test of child s of a node; returns 1 on overlap of
  * [tmin, tcur]; *tn receives the entry distance. */
 static int box_test(const orc_node* nd, int s, const oray* r, float tmin, float tcur, float* tn) {
-    /* near/far planes by the sign bit of the direction (-0 selects like < 0,
-       matching 1/-0 = -inf) */
-    float nx = signbit(r->d.x) ? nd->hi_x[s] : nd->lo_x[s];
-    float fx = signbit(r->d.x) ? nd->lo_x[s] : nd->hi_x[s];
-    float ny = signbit(r->d.y) ? nd->hi_y[s] : nd->lo_y[s];
-    float fy = signbit(r->d.y) ? nd->lo_y[s] : nd->hi_y[s];
-    float nz = signbit(r->d.z) ? nd->hi_z[s] : nd->lo_z[s];
-    float fz = signbit(r->d.z) ? nd->lo_z[s] : nd->hi_z[s];
+    /* near/far planes by the sign bit of the inverse direction (1/-0 = -inf
+       selects like a negative direction; for a NaN component both planes
+       give NaN, so the choice does not matter) */
+    float nx = signbit(r->inv.x) ? nd->hi_x[s] : nd->lo_x[s];
+    float fx = signbit(r->inv.x) ? nd->lo_x[s] : nd->hi_x[s];
+    float ny = signbit(r->inv.y) ? nd->hi_y[s] : nd->lo_y[s];
+    float fy = signbit(r->inv.y) ? nd->lo_y[s] : nd->hi_y[s];
+    float nz = signbit(r->inv.z) ? nd->hi_z[s] : nd->lo_z[s];
+    float fz = signbit(r->inv.z) ? nd->lo_z[s] : nd->hi_z[s];
     float tnx = (nx - r->o.x) * r->inv.x, tfx = (fx - r->o.x) * r->inv.x;
     float tny = (ny - r->o.y) * r->inv.y, tfy = (fy - r->o.y) * r->inv.y;
     float tnz = (nz - r->o.z) * r->inv.z, tfz = (fz - r->o.z) * r->inv.z;

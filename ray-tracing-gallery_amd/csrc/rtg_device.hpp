@@ -434,7 +434,8 @@ __device__ inline bool tri_test(const RayCtx& r, V3 p0, V3 p1, V3 p2, float tmin
 }
 
 // Robust slab test of one slot (Ize 2013): near and far planes chosen by
-// the sign bit of the ray direction; the far distance is inflated by
+// the sign bit of the inverse direction (1/-0 = -inf counts as negative; a
+// NaN component makes both planes NaN); the far distance is inflated by
 // 1 + 2 gamma(3) before the comparison.
 __device__ inline bool slab(float lx, float hx, float ly, float hy, float lz, float hz, const RayCtx& r, bool sx,
                             bool sy, bool sz, float tmin, float tcur, float& tn) {
@@ -614,7 +615,7 @@ struct Traverser {
             float4 LZ = gld_f4(np + 4), HZ = gld_f4(np + 5);
             uint4 C = gld_u4(np + 6);
             if (COUNT) cnt.nodes++;
-            const bool sx = signbit(cr.d.x), sy = signbit(cr.d.y), sz = signbit(cr.d.z);
+            const bool sx = signbit(cr.inv.x), sy = signbit(cr.inv.y), sz = signbit(cr.inv.z);
             float t0, t1, t2, t3;
             bool h0 = slab(LX.x, HX.x, LY.x, HY.x, LZ.x, HZ.x, cr, sx, sy, sz, tmin, tcur, t0) && C.x != kEmpty;
             bool h1 = slab(LX.y, HX.y, LY.y, HY.y, LZ.y, HZ.y, cr, sx, sy, sz, tmin, tcur, t1) && C.y != kEmpty;

@@ -314,7 +314,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int le
     uint32_t slot = 0, path = 0;
     const JobDev* J = nullptr;
     TlasView T;
-    uint64_t c0 = 0;
+    uint32_t c0 = 0;
     while (true) {
         uint64_t idle = __ballot(!active);
         if (want_refill(idle, feeder, F.refill_closest)) {
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int le
                 T = tlas_view(J->tlas);
                 float4 a = Q.o[id], b = Q.d[id];
                 tr.start(T, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, RTG_RAY_FLAG_NONE, 0xffu, 0u, st);
-                if (J->flags & RTG_TRACE_HEATMAP) c0 = __builtin_readcyclecounter();
+                if (J->flags & RTG_TRACE_HEATMAP) c0 = (uint32_t)__builtin_readcyclecounter();
                 active = true;
             }
         }
@@ -339,7 +339,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int le
             int res = tr.step(T, J->scene, st, cnt);
             if (res != kRayActive) {
                 if (res == kRayOverflow) atomicOr(status, (uint32_t)kStatusStackOverflow);
-                if (J->flags & RTG_TRACE_HEATMAP) atomicAdd(&F.cycles[path], (uint32_t)(__builtin_readcyclecounter() - c0));
+                if (J->flags & RTG_TRACE_HEATMAP) atomicAdd(&F.cycles[path], (uint32_t)__builtin_readcyclecounter() - c0);
                 const HitRec& h = tr.best;
                 F.hit_a[slot] = make_uint4(f2u(h.t), h.inst, h.prim, h.geom);
                 F.hit_b[slot] = make_float2(h.u, h.v);
@@ -705,7 +705,7 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5
     uint32_t path = 0;
     const JobDev* J = nullptr;
     TlasView T;
-    uint64_t c0 = 0;
+    uint32_t c0 = 0;
     while (true) {
         uint64_t idle = __ballot(!active);
         if (want_refill(idle, feeder, F.refill_occlusion)) {
@@ -722,7 +722,7 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5
                 float4 a = F.sq.o[id], b = F.sq.d[id];
                 tr.start(T, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w,
                          RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RTG_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER, 0xffu, 1u, st);
-                if (J->flags & RTG_TRACE_HEATMAP) c0 = __builtin_readcyclecounter();
+                if (J->flags & RTG_TRACE_HEATMAP) c0 = (uint32_t)__builtin_readcyclecounter();
                 active = true;
             }
         }
@@ -731,7 +731,7 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5
             int res = tr.step(T, J->scene, st, cnt);
             if (res != kRayActive) {
                 if (res == kRayOverflow) atomicOr(status, (uint32_t)kStatusStackOverflow);
-                if (J->flags & RTG_TRACE_HEATMAP) atomicAdd(&F.cycles[path], (uint32_t)(__builtin_readcyclecounter() - c0));
+                if (J->flags & RTG_TRACE_HEATMAP) atomicAdd(&F.cycles[path], (uint32_t)__builtin_readcyclecounter() - c0);
                 // shadow_ray_miss (lib.rs:33-36): a miss counts as visible
                 if (res == kRayMiss) atomicAdd(&F.vis[path], 1u);
                 active = false;
