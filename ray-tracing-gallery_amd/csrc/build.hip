@@ -754,9 +754,11 @@ __global__ __launch_bounds__(256) void k_gather_insts(const uint32_t* order, con
 
 // In-place TLAS UPDATE (src/util_structs.rs:285-357): new instance records,
 // same topology. One workgroup walks the level lists from the deepest level
-// to the root with a barrier between levels.
-__global__ __launch_bounds__(1024) void k_tlas_refit(uint8_t* base, const rtg_instance* inst, uint32_t n,
-                                                     const uint64_t* reg, uint32_t nreg, uint32_t* status) {
+// to the root with a barrier between levels. 256 lanes, one wave per SIMD:
+// while another frame's traversal waves fill the GPU, such a workgroup finds
+// room on a CU long before a 1024-lane one (four waves per SIMD) would.
+__global__ __launch_bounds__(256) void k_tlas_refit(uint8_t* base, const rtg_instance* inst, uint32_t n,
+                                                    const uint64_t* reg, uint32_t nreg, uint32_t* status) {
     AsHeader* h = reinterpret_cast<AsHeader*>(base);
     Node* nodes = reinterpret_cast<Node*>(base + h->nodes_off);
     InstRec* recs = reinterpret_cast<InstRec*>(base + h->prims_off);
@@ -994,7 +996,7 @@ int refresh_registry(rtg_ctx* ctx) {
 
 int launch_refit(rtg_ctx* ctx, uint8_t* tlas, const rtg_instance* instances, uint32_t count, hipStream_t s) {
     LaunchTimer timer(ctx, RTG_KERNEL_REFIT, s);
-    hipLaunchKernelGGL(k_tlas_refit, dim3(1), dim3(1024), 0, s, tlas, instances, count, ctx->d_registry,
+    hipLaunchKernelGGL(k_tlas_refit, dim3(1), dim3(256), 0, s, tlas, instances, count, ctx->d_registry,
                        ctx->registry_count, ctx->d_status);
     RTG_HIP(hipGetLastError());
     return RTG_OK;
