@@ -682,7 +682,7 @@ __device__ inline void shadow_part(const Frame& F, int part, uint32_t& lo, uint3
 
 // At most 96 VGPRs (five waves per SIMD): beside four waves of this kernel a
 // SIMD then still holds one wave of the other frame in flight (closest-hit
-// traversal, 122 VGPRs, or shading, 116), which keeps the GPU fuller than
+// traversal, 118 VGPRs, or shading, 116), which keeps the GPU fuller than
 // the four-wave allocation of 102 VGPRs did.
 template <bool COUNT, bool SIDE>
 __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_trace_occlusion(
