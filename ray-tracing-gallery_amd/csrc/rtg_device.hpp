@@ -566,8 +566,7 @@ struct Traverser {
     uint32_t inst_index, inst_flags;
     uint32_t cur;
     bool in_blas;
-    int found;
-    HitRec best;
+    HitRec best;  // best.inst stays RTG_NO_HIT until a hit is accepted
 
     __device__ inline void start(const TlasView& T, V3 o, V3 d, float tmin_, float tmax_, uint32_t flags,
                                  uint32_t mask, uint32_t sbt, Stack& st) {
@@ -581,7 +580,6 @@ struct Traverser {
         cull_mask = mask;
         sbt_offset = sbt;
         inst_index = inst_flags = 0;
-        found = 0;
         best.t = tmax_;
         best.u = best.v = 0.0f;
         best.inst = best.prim = best.geom = RTG_NO_HIT;
@@ -589,7 +587,7 @@ struct Traverser {
         cur = 0;
     }
 
-    __device__ inline int done() const { return found ? kRayHit : kRayMiss; }
+    __device__ inline int done() const { return best.inst != RTG_NO_HIT ? kRayHit : kRayMiss; }
 
     // Next entry to visit; false once the stack is empty.
     __device__ inline bool pop(const TlasView& T, Stack& st) {
@@ -702,7 +700,6 @@ struct Traverser {
                     if (!alpha_clip_keep(S, T.src, inst_index, geom, prim, u, v)) continue;
                 }
             }
-            found = 1;
             best.t = t;
             best.u = u;
             best.v = v;

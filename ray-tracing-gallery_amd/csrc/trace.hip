@@ -294,8 +294,11 @@ __global__ __launch_bounds__(256) void k_raygen(Frame F, bool count) {
 }
 
 // Closest-hit traversal of trace level b (count from the device unless fixed).
-// SIDE instances run the bounce levels on the bounce stream.
-template <bool COUNT, bool SIDE>
+// SIDE instances run the bounce levels on the bounce stream. SINGLE instances
+// serve launches of one job: its TLAS view and scene stay in scalar registers
+// instead of per lane, which brings the kernel to 104 VGPRs, so that a wave
+// of the other frame's occlusion kernel (96) fits beside four of its waves.
+template <bool COUNT, bool SIDE, bool SINGLE>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int level, const uint32_t* count_ptr,
                                                                  uint32_t fixed_count, uint32_t* status) {
     __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
@@ -311,9 +314,10 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int le
     WaveFeeder feeder;
     feeder.init(F.feed_chunk);
     bool active = false;
-    uint32_t slot = 0, path = 0;
-    const JobDev* J = nullptr;
+    uint32_t slot = 0;
+    const JobDev* J = SINGLE ? &F.jobs[0] : nullptr;
     TlasView T;
+    if (SINGLE) T = tlas_view(J->tlas);
     uint32_t c0 = 0;
     while (true) {
         uint64_t idle = __ballot(!active);
@@ -321,13 +325,15 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int le
             uint32_t id = feeder.feed(idle, !active, &F.counts[kFeedLevel + level], n);
             if (id != ~0u) {
                 slot = id;
-                path = Q.path[id];
+                uint32_t path = Q.path[id];
                 if (path >= F.paths) {
                     queue_range_error(F);
                     path = 0;  // traced as path 0's ray; the shade stage drops it
                 }
-                J = &F.jobs[find_job(F.jobs, F.njobs, path, false)];
-                T = tlas_view(J->tlas);
+                if (!SINGLE) {
+                    J = &F.jobs[find_job(F.jobs, F.njobs, path, false)];
+                    T = tlas_view(J->tlas);
+                }
                 float4 a = Q.o[id], b = Q.d[id];
                 tr.start(T, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, RTG_RAY_FLAG_NONE, 0xffu, 0u, st);
                 if (J->flags & RTG_TRACE_HEATMAP) c0 = (uint32_t)__builtin_readcyclecounter();
@@ -339,7 +345,11 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int le
             int res = tr.step(T, J->scene, st, cnt);
             if (res != kRayActive) {
                 if (res == kRayOverflow) atomicOr(status, (uint32_t)kStatusStackOverflow);
-                if (J->flags & RTG_TRACE_HEATMAP) atomicAdd(&F.cycles[path], (uint32_t)__builtin_readcyclecounter() - c0);
+                if (J->flags & RTG_TRACE_HEATMAP) {
+                    // the path is read again rather than kept live through the traversal
+                    const uint32_t path = Q.path[slot];
+                    atomicAdd(&F.cycles[path < F.paths ? path : 0u], (uint32_t)__builtin_readcyclecounter() - c0);
+                }
                 const HitRec& h = tr.best;
                 F.hit_a[slot] = make_uint4(f2u(h.t), h.inst, h.prim, h.geom);
                 F.hit_b[slot] = make_float2(h.u, h.v);
@@ -682,7 +692,7 @@ __device__ inline void shadow_part(const Frame& F, int part, uint32_t& lo, uint3
 
 // At most 96 VGPRs (five waves per SIMD): beside four waves of this kernel a
 // SIMD then still holds one wave of the other frame in flight (closest-hit
-// traversal, 118 VGPRs, or shading, 116), which keeps the GPU fuller than
+// traversal, 104 VGPRs, or shading, 116), which keeps the GPU fuller than
 // the four-wave allocation of 102 VGPRs did.
 template <bool COUNT, bool SIDE>
 __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_trace_occlusion(
@@ -1141,6 +1151,7 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
         {
             LaunchTimer t(ctx, RTG_KERNEL_TRACE_CLOSEST, st, tm);
             const bool side = Fl.side != 0;
+            const bool single = Fl.njobs == 1;
             if (tune.refill_closest == 0) {
                 if (side)
                     hipLaunchKernelGGL((cnt ? k_trace_closest_rounds<true, true> : k_trace_closest_rounds<false, true>),
@@ -1148,12 +1159,12 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
                 else
                     hipLaunchKernelGGL((cnt ? k_trace_closest_rounds<true, false> : k_trace_closest_rounds<false, false>),
                                        dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, ctx->d_status);
-            } else if (side) {
-                hipLaunchKernelGGL((cnt ? k_trace_closest<true, true> : k_trace_closest<false, true>), dim3(g),
-                                   dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, ctx->d_status);
             } else {
-                hipLaunchKernelGGL((cnt ? k_trace_closest<true, false> : k_trace_closest<false, false>), dim3(g),
-                                   dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, ctx->d_status);
+                auto kernel = side ? (cnt ? (single ? k_trace_closest<true, true, true> : k_trace_closest<true, true, false>)
+                                          : (single ? k_trace_closest<false, true, true> : k_trace_closest<false, true, false>))
+                                   : (cnt ? (single ? k_trace_closest<true, false, true> : k_trace_closest<true, false, false>)
+                                          : (single ? k_trace_closest<false, false, true> : k_trace_closest<false, false, false>));
+                hipLaunchKernelGGL(kernel, dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, ctx->d_status);
             }
         }
         {
