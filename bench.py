@@ -448,8 +448,12 @@ def main():
     elapsed = t1 - t0
     value = rays / elapsed / 1e6
 
+    # the dominant traversal kernel: the one whose frame-stream launches move
+    # the most algorithmic bytes (the level-0 shadow rays, three quarters of
+    # the rays); by time the two kernels are close, and which one is longer
+    # depends on how the two frames in flight interleave
     kinds = {"closest": ktime["trace_closest"], "occlusion": ktime["trace_occlusion"]}
-    dom = max(kinds, key=lambda k: kinds[k][0])
+    dom = max(kinds, key=lambda k: algorithmic_bytes(tot, k))
     ms, launches = kinds[dom]
     bytes_total = algorithmic_bytes(tot, dom)
     achieved = bytes_total / (ms / 1e3) / 1e9 if ms > 0 else 0.0
