@@ -20,12 +20,14 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "ray-tracing-gallery_amd"))
 
 # HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues per process (4 by
-# default). This process holds the frame path's bounce stream and a server
-# with three batches in flight on three streams; with 4 queues two busy
-# streams end up on one queue and serialise (served 446 -> 396 requests/s,
-# measured). Raised to 8 before the HIP runtime starts.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+# default). This process holds two frames in flight (a stream and a bounce
+# stream each) and servers with up to four batches in flight (again two
+# streams each); when busy streams share a queue they serialise. Measured on
+# MI355X, served mode: 4 queues 396, 8 queues 446-466, 16 queues 516
+# requests/s; the frame rate is the same with 8 and 16. Raised to 16 before
+# the HIP runtime starts.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "Mrays/s at 1080p 2M-tri scene; p50/p99 request latency, served vs plain"
@@ -99,10 +101,10 @@ def cpu_baseline(scene, threads, min_seconds=10.0):
 
 # (requests per batch, batches in flight, samples per batch): plain = one
 # request per launch, one at a time; served = continuous batching with up to
-# 4 batches in flight of at most 1 M samples each (the setting with the best
-# p99 in a sweep on MI355X with 8 hardware queues; larger batches delay small
-# requests)
-SERVE_MODES = {"plain": (1, 1, 0), "served": (32, 4, 1 << 20)}
+# 5 batches in flight of at most 1 M samples each (the setting with the best
+# throughput and p99 in a sweep on MI355X with 16 hardware queues; larger
+# batches delay small requests)
+SERVE_MODES = {"plain": (1, 1, 0), "served": (32, 5, 1 << 20)}
 
 
 def serving_benchmark(ctx, n_scenes=32, n_requests=160, seed=0x5EED0004, modes=None):

@@ -125,11 +125,7 @@ bool rtg_server::launch(Slot& s) {
         j.out_rgb = r.out_rgb;
         j.out_primary_hits = r.out_primary_hits;
     }
-    // with several batches in flight their overlap keeps the GPU busy
-    // through the short bounce levels: each batch runs its levels in sequence
-    Tuning t = ctx->tune;
-    if (slots.size() > 1) t.bounce_blocks = 0;
-    if (trace_jobs(ctx, jobs.data(), (uint32_t)jobs.size(), s.stream, s.arena, false, &t) != RTG_OK) return false;
+    if (trace_jobs(ctx, jobs.data(), (uint32_t)jobs.size(), s.stream, s.arena, false) != RTG_OK) return false;
     return hipEventRecord(s.done, s.stream) == hipSuccess;
 }
 
