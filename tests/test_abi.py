@@ -91,3 +91,43 @@ def test_tuning_layout():
     t = abi.Tuning.make()
     assert t.as_dict() == {"blocks_per_cu": 4, "feed_chunk": 64, "refill_closest": 64, "refill_occlusion": 64,
                            "path_order": 1, "bounce_blocks": 128}
+
+
+MIRRORS = {
+    "rtg_instance": abi.Instance, "rtg_uniforms": abi.Uniforms, "rtg_model_info": abi.ModelInfo,
+    "rtg_geometry_info": abi.GeometryInfo, "rtg_texture": abi.Texture, "rtg_geometry": abi.Geometry,
+    "rtg_build_sizes": abi.BuildSizes, "rtg_as_info": abi.AsInfo, "rtg_ray": abi.Ray, "rtg_hit": abi.Hit,
+    "rtg_scene_bindings": abi.SceneBindings, "rtg_trace_opts": abi.TraceOpts, "rtg_image_job": abi.ImageJob,
+    "rtg_trace_counters": abi.TraceCounters, "rtg_tuning": abi.Tuning, "rtg_server_config": abi.ServerConfig,
+    "rtg_request": abi.Request, "rtg_completion": abi.Completion, "rtg_server_stats": abi.ServerStats,
+}
+
+
+def c_fields(cname):
+    """Field names of a struct in rtg.h, in declaration order."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (cname, cname), text, re.S).group(1)
+    return [re.search(r"(\w+)\s*(\[[^\]]*\])*\s*$", d.strip()).group(1) for d in body.split(";") if d.strip()]
+
+
+def test_ctypes_mirrors_match_the_c_compiler(tmp_path):
+    """Every ctypes mirror has the field names, offsets and size that gcc
+    gives the struct in include/rtg.h."""
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "rtg.h"', "int main(void) {"]
+    for cname in MIRRORS:
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f in c_fields(cname):
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(REPO, "include"), "-o", str(exe), str(src)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                        text=True).stdout.splitlines())
+    for cname, py in MIRRORS.items():
+        names = [n for n, _ in py._fields_]
+        assert names == c_fields(cname), cname
+        assert C.sizeof(py) == int(got[cname]), cname
+        for n in names:
+            assert getattr(py, n).offset == int(got[f"{cname}.{n}"]), (cname, n)
