@@ -98,9 +98,11 @@ struct Tuning {
     int feed_chunk = 64;
     // persistent blocks that trace the bounce levels beside the first
     // level's shadow rays (0: all levels in sequence on the caller's stream);
-    // 128 of 1,024 measured best on the 1080p scene (64 leaves the bounce
-    // chain on the critical path, 256 and more slow the shadow rays)
-    int bounce_blocks = 128;
+    // 192 of 1,024 measured best on the 1080p scene with two frames in
+    // flight: with 128 or 160 the bounce chain, slowed by the other frame's
+    // closest-hit traversal, fell onto the critical path in some runs; 256
+    // and more slow the shadow rays
+    int bounce_blocks = 192;
 };
 
 }  // namespace rtg

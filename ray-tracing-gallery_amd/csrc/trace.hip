@@ -296,12 +296,12 @@ __global__ __launch_bounds__(256) void k_raygen(Frame F, bool count) {
 // Closest-hit traversal of trace level b (count from the device unless fixed).
 // SIDE instances run the bounce levels on the bounce stream. SINGLE instances
 // serve launches of one job: its TLAS view and scene stay in scalar registers
-// instead of per lane, which brings the kernel to 104 VGPRs, so that a wave
-// of the other frame's occlusion kernel (96) fits beside four of its waves.
+// instead of per lane, which lets k_trace_closest_one fit 96 VGPRs without
+// spills, five waves per SIMD like the occlusion kernel, so any mix of the
+// two frames' traversal waves fills a SIMD.
 template <bool COUNT, bool SIDE, bool SINGLE>
-__global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int level, const uint32_t* count_ptr,
-                                                                 uint32_t fixed_count, uint32_t* status) {
-    __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
+__device__ __forceinline__ void trace_closest(Frame F, int level, const uint32_t* count_ptr, uint32_t fixed_count,
+                                              uint32_t* status, uint2* s_stack) {
     const uint32_t n = min(count_ptr ? *count_ptr : fixed_count, F.path_cap);
     const Queue Q = F.q[level & 1];
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
@@ -358,6 +358,23 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int le
         }
     }
     if (COUNT) flush_counters(F.counters, cnt, SIDE ? kCountersClosestSide : kCountersClosest);
+}
+
+// Launches of one job (held to 96 VGPRs, five waves per SIMD).
+template <bool COUNT, bool SIDE>
+__global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_trace_closest_one(
+    Frame F, int level, const uint32_t* count_ptr, uint32_t fixed_count, uint32_t* status) {
+    __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
+    trace_closest<COUNT, SIDE, true>(F, level, count_ptr, fixed_count, status, s_stack);
+}
+
+// Launches of several jobs (batches): the job of each ray is looked up; at
+// 96 VGPRs this variant would spill, so it keeps four waves per SIMD.
+template <bool COUNT, bool SIDE>
+__global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int level, const uint32_t* count_ptr,
+                                                                 uint32_t fixed_count, uint32_t* status) {
+    __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
+    trace_closest<COUNT, SIDE, false>(F, level, count_ptr, fixed_count, status, s_stack);
 }
 
 // Non-persistent variant: lanes take rays in grid-stride rounds and run each
@@ -692,7 +709,7 @@ __device__ inline void shadow_part(const Frame& F, int part, uint32_t& lo, uint3
 
 // At most 96 VGPRs (five waves per SIMD): beside four waves of this kernel a
 // SIMD then still holds one wave of the other frame in flight (closest-hit
-// traversal, 104 VGPRs, or shading, 116), which keeps the GPU fuller than
+// traversal, 96 VGPRs, or shading, 116), which keeps the GPU fuller than
 // the four-wave allocation of 102 VGPRs did.
 template <bool COUNT, bool SIDE>
 __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_trace_occlusion(
@@ -1160,10 +1177,10 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
                     hipLaunchKernelGGL((cnt ? k_trace_closest_rounds<true, false> : k_trace_closest_rounds<false, false>),
                                        dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, ctx->d_status);
             } else {
-                auto kernel = side ? (cnt ? (single ? k_trace_closest<true, true, true> : k_trace_closest<true, true, false>)
-                                          : (single ? k_trace_closest<false, true, true> : k_trace_closest<false, true, false>))
-                                   : (cnt ? (single ? k_trace_closest<true, false, true> : k_trace_closest<true, false, false>)
-                                          : (single ? k_trace_closest<false, false, true> : k_trace_closest<false, false, false>));
+                auto kernel = side ? (cnt ? (single ? k_trace_closest_one<true, true> : k_trace_closest<true, true>)
+                                          : (single ? k_trace_closest_one<false, true> : k_trace_closest<false, true>))
+                                   : (cnt ? (single ? k_trace_closest_one<true, false> : k_trace_closest<true, false>)
+                                          : (single ? k_trace_closest_one<false, false> : k_trace_closest<false, false>));
                 hipLaunchKernelGGL(kernel, dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, ctx->d_status);
             }
         }
