@@ -404,3 +404,140 @@ def test_treelet_structures_give_the_same_frame():
     assert np.array_equal(fast[2].view(np.uint32), plain[2].view(np.uint32))
     assert fast[3].node_visits < plain[3].node_visits
     assert fast[3].occ_node_visits < plain[3].occ_node_visits
+
+
+TREELET_LEAVES = 7
+TREELET_PASSES = 5
+
+
+def treelet_reference(lc, rc, leaf_boxes, passes=TREELET_PASSES):
+    """Treelet restructuring (Karras & Aila 2013, section 4) restated from the
+    method's description, applied to the plain LBVH's binary tree (lc, rc over
+    internal nodes 0..n-2, LEAF | i for sorted primitive i):
+
+    * each pass visits every internal node after all nodes below it;
+    * the treelet of node r starts as r's two children; while it has fewer
+      than seven leaves, its internal leaf of largest float32 box area (the
+      first in treelet order on ties) is opened: its left child takes its
+      place and its right child is appended;
+    * the treelet gets the topology of least summed internal-node area: over
+      leaf subsets S (bit k = treelet leaf k), cost(S) = area(S) + the least
+      cost(P) + cost(S \\ P) over subsets P holding S's lowest leaf, tried in
+      decreasing P and keeping the first least; single leaves cost 0; all in
+      float32;
+    * node numbers are handed out from a stack: r for the whole set, then the
+      opened nodes in the order they were opened, the left part of a split
+      before the right, the left part's subtree first;
+    * boxes of the treelet's nodes are then recomputed children first.
+    Returns the restructured (lc, rc)."""
+    lc = [int(x) for x in lc]
+    rc = [int(x) for x in rc]
+    n_int = len(lc)
+    f = np.float32
+
+    def box_of(c, boxes):
+        return leaf_boxes[c & ~LEAF] if c & LEAF else boxes[c]
+
+    def union(a, b):
+        return np.concatenate([np.minimum(a[:3], b[:3]), np.maximum(a[3:], b[3:])]).astype(np.float32)
+
+    def area(b):
+        return f32_area(b[:3], b[3:])
+
+    # boxes of the internal nodes, children first
+    boxes = [None] * n_int
+
+    def fill(i):
+        for c in (lc[i], rc[i]):
+            if not c & LEAF:
+                fill(c)
+        boxes[i] = union(box_of(lc[i], boxes), box_of(rc[i], boxes))
+
+    fill(0)
+
+    def postorder(i, out):
+        for c in (lc[i], rc[i]):
+            if not c & LEAF:
+                postorder(c, out)
+        out.append(i)
+        return out
+
+    for _ in range(passes):
+        for r in postorder(0, []):
+            leaves, inner = [lc[r], rc[r]], []
+            while len(leaves) < TREELET_LEAVES:
+                best = None
+                for k, c in enumerate(leaves):
+                    if not c & LEAF and (best is None or area(boxes[c]) > area(boxes[leaves[best]])):
+                        best = k
+                if best is None:
+                    break
+                c = leaves[best]
+                inner.append(c)
+                leaves[best] = lc[c]
+                leaves.append(rc[c])
+            m = len(leaves)
+            if m < 3:
+                continue
+            lb = [box_of(c, boxes) for c in leaves]
+            cost, split = {}, {}
+            for S in range(1, 1 << m):
+                if S & (S - 1) == 0:
+                    cost[S] = f(0.0)
+                    continue
+                b = None
+                for k in range(m):
+                    if S >> k & 1:
+                        b = lb[k] if b is None else union(b, lb[k])
+                low = S & -S
+                best_c, best_p = None, None
+                P = (S - 1) & S
+                while P:
+                    if P & low:
+                        c = f(cost[P] + cost[S ^ P])
+                        if best_c is None or c < best_c:
+                            best_c, best_p = c, P
+                    P = (P - 1) & S
+                cost[S] = f(area(b) + best_c)
+                split[S] = best_p
+            stack, order, nxt = [((1 << m) - 1, r)], [], 0
+            while stack:
+                S, node = stack.pop()
+                order.append(node)
+                parts = [split[S], S ^ split[S]]
+                refs = []
+                for p in parts:
+                    if p & (p - 1):
+                        refs.append(inner[nxt])
+                        nxt += 1
+                    else:
+                        refs.append(leaves[p.bit_length() - 1])
+                lc[node], rc[node] = refs
+                for h in (1, 0):
+                    if parts[h] & (parts[h] - 1):
+                        stack.append((parts[h], refs[h]))
+            for node in reversed(order):
+                boxes[node] = union(box_of(lc[node], boxes), box_of(rc[node], boxes))
+    return np.array(lc, np.uint32), np.array(rc, np.uint32)
+
+
+@pytest.mark.parametrize("kind,n,seed", [("random", 24, 1), ("random", 61, 2), ("random", 90, 3),
+                                         ("grid", 64, 0), ("torus", 0, 0)])
+def test_treelet_passes_match_a_restatement_node_for_node(kind, n, seed):
+    """The oracle's five treelet passes give, node for node, the tree that an
+    independent restatement of the method (treelet_reference) builds from the
+    oracle's plain LBVH over the same primitives: which nodes form each
+    treelet, its optimal topology, the tie-breaks and the node numbering."""
+    if kind == "random":
+        boxes = _random_boxes(n, seed)
+    elif kind == "grid":  # equal boxes: ties in every area comparison
+        g = np.stack(np.meshgrid(*[np.arange(4, dtype=np.float32)] * 3), -1).reshape(-1, 3)
+        boxes = np.concatenate([g, g + 0.5], 1).astype(np.float32)
+    else:
+        boxes = tri_boxes(scenes.torus(8, 6)).astype(np.float32)
+    plain_lc, plain_rc, order = oh.binary_tree(boxes, 0)
+    lc, rc, order_fast = oh.binary_tree(boxes, FAST)
+    assert np.array_equal(order, order_fast)
+    ref_lc, ref_rc = treelet_reference(plain_lc, plain_rc, boxes[order])
+    assert np.array_equal(lc, ref_lc)
+    assert np.array_equal(rc, ref_rc)
