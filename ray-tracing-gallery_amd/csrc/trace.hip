@@ -171,7 +171,7 @@ __device__ inline void flush_counters(uint64_t* counters, const TravCounters& c,
 // empty: a lane that finishes asks the wave's feeder for the next entry once
 // at least `refill_min` lanes are idle (or all are).
 __device__ inline bool want_refill(uint64_t idle, const WaveFeeder& f, int refill_min) {
-    return idle != 0 && !f.exhausted && (__popcll(idle) >= refill_min || idle == ~0ull);
+    return idle != 0 && !f.exhausted && ((int)__popcll(idle) >= refill_min || idle == ~0ull);
 }
 
 // ---------------------------------------------------------------------------
