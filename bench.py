@@ -285,6 +285,43 @@ def batched_benchmark(ctx, steps, seed=0x5EED0003):
             "graph_vs_plain": round(t_plain / t_graph, 3)}
 
 
+def blas_build_times(ctx, ds, scene, reps=2):
+    """Wall time of rtg_blas_build (synchronous) over the scene's first mesh,
+    already resident on the device, as the plain LBVH and with
+    PREFER_FAST_TRACE (five treelet passes); the best of `reps` builds each."""
+    import ctypes as C
+    import torch
+    from rtg import abi
+    from rtg.lib import check, load
+    L = load()
+    m = scene.meshes[0]
+    pos = ds.mesh_bufs[0][0]
+    idx = ds.mesh_bufs[0][3]
+    geoms = (abi.Geometry * len(m.geometries))()
+    for g, (ix, op) in enumerate(m.geometries):
+        geoms[g].vertex_data = pos.data_ptr()
+        geoms[g].vertex_stride = 12
+        geoms[g].max_vertex = max(0, len(m.positions) - 1)
+        geoms[g].index_data = idx[g].data_ptr()
+        geoms[g].index_count = ix.size
+        geoms[g].flags = abi.RTG_GEOMETRY_OPAQUE if op else 0
+    out = {"triangles": int(sum(ix.shape[0] for ix, _ in m.geometries))}
+    for name, flags in (("plain", 0), ("fast_trace", abi.RTG_BUILD_PREFER_FAST_TRACE)):
+        best = None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            h = C.c_uint64()
+            t = time.perf_counter()
+            check(L.rtg_blas_build(ctx.ptr, C.cast(geoms, C.c_void_p), len(m.geometries), flags, C.byref(h), None),
+                  "rtg_blas_build")
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t) * 1e3
+            L.rtg_blas_destroy(ctx.ptr, h.value)
+            best = ms if best is None else min(best, ms)
+        out[name] = round(best, 2)
+    return out
+
+
 def incoherent_benchmark(ctx, n_rays=16 << 20, sample=65536):
     """Incoherent-ray stress: one BLAS of 1,000,000 random triangles
     (lognormal edge lengths, median 0.01) in the unit cube; 16M rays with
@@ -353,6 +390,7 @@ def incoherent_benchmark(ctx, n_rays=16 << 20, sample=65536):
                                                                    np.asarray(o_occ).astype(np.uint32)))}
     out["cpu_oracle_mrays_s"] = round(2 * sample / cpu_s / 1e6, 3)
     out["cpu_cores"] = threads
+    out["blas_build_ms"] = blas_build_times(ctx, ds, scene)
     ds.destroy()
     del hits, occ, rd
     torch.cuda.empty_cache()
