@@ -147,7 +147,7 @@ extern "C" int rtg_autotune(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t n,
         {64, 48, 32, 0},      // refill_closest
         {64, 48, 32, 0},      // refill_occlusion
         {1, 0},               // path_order
-        {128, 256, 64, 0},    // bounce_blocks
+        {192, 128, 256, 64, 0},  // bounce_blocks
     };
     std::map<Tuning, double> seen;
     Tuning cur = ctx->tune;
