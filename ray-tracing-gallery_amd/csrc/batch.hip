@@ -38,8 +38,10 @@ static int check_tuning(const rtg_tuning& t) {
     return RTG_OK;
 }
 
-static Tuning from_abi(const rtg_tuning& t) {
+// rtg_tuning leaves out the ray-query refill threshold, which is kept
+static Tuning from_abi(const rtg_tuning& t, const Tuning& current) {
     Tuning u;
+    u.refill_rays = current.refill_rays;
     u.blocks_per_cu = (int)t.blocks_per_cu;
     u.feed_chunk = (int)t.feed_chunk;
     u.refill_closest = (int)t.refill_closest;
@@ -121,7 +123,7 @@ extern "C" int rtg_set_tuning(rtg_ctx* ctx, const rtg_tuning* tuning) {
     RTG_REQUIRE(ctx && tuning, RTG_ERROR_INVALID_ARGUMENT, "rtg_set_tuning: null argument");
     int st = check_tuning(*tuning);
     if (st != RTG_OK) return st;
-    ctx->tune = from_abi(*tuning);
+    ctx->tune = from_abi(*tuning, ctx->tune);
     return RTG_OK;
 }
 

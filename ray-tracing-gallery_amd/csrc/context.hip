@@ -111,6 +111,7 @@ extern "C" int rtg_create(rtg_ctx** out, int device) {
     };
     ctx->tune.refill_closest = env_int("RTG_REFILL_CLOSEST", ctx->tune.refill_closest, 0, 64);
     ctx->tune.refill_occlusion = env_int("RTG_REFILL_OCCLUSION", ctx->tune.refill_occlusion, 0, 64);
+    ctx->tune.refill_rays = env_int("RTG_REFILL_RAYS", ctx->tune.refill_rays, 0, 64);
     ctx->tune.path_order = env_int("RTG_PATH_ORDER", ctx->tune.path_order, 0, 1);
     ctx->tune.blocks_per_cu = env_int("RTG_TRACE_BLOCKS_PER_CU", ctx->tune.blocks_per_cu, 1, 8);
     // RTG_FEED_CHUNK counts 64-ray batches taken per atomic

@@ -83,8 +83,8 @@ struct FrameArena;  // trace.hip
 
 // Launch parameters of the traversal kernels (defaults, overridable through
 // the environment: RTG_REFILL_CLOSEST, RTG_REFILL_OCCLUSION (0 selects the
-// non-persistent kernels), RTG_PATH_ORDER, RTG_TRACE_BLOCKS_PER_CU,
-// RTG_FEED_CHUNK, RTG_BOUNCE_BLOCKS).
+// non-persistent kernels), RTG_REFILL_RAYS, RTG_PATH_ORDER,
+// RTG_TRACE_BLOCKS_PER_CU, RTG_FEED_CHUNK, RTG_BOUNCE_BLOCKS).
 // Measured on MI355X (1080p large scene, 4-wide BVH): refilling lanes before
 // the whole wave is idle loses 13-16 % on the occlusion kernel, so the
 // default refills only an idle wave, one 64-ray batch per atomic; four
@@ -93,6 +93,14 @@ struct FrameArena;  // trace.hip
 struct Tuning {
     int refill_closest = 64;
     int refill_occlusion = 64;
+    // idle lanes before a wavefront of rtg_trace_rays takes new rays (both
+    // modes). Not part of rtg_tuning: a frame's rays start together and end
+    // alike, but incoherent queries leave waves with a few long rays, and
+    // refilling at 16 idle lanes measured 1,384/2,479 Mrays/s (closest /
+    // occlusion; 16M rays, origins uniform in the unit cube, directions
+    // uniform on the sphere, into 1M random triangles) against 1,006/1,604
+    // at 64; 8 and 24 were within 4 %
+    int refill_rays = 16;
     int path_order = 1;
     int blocks_per_cu = 4;
     int feed_chunk = 64;

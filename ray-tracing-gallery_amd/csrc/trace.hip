@@ -1322,7 +1322,7 @@ extern "C" int rtg_trace_rays(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindi
 #define RTG_LAUNCH_RAYS(OCC, CNT)                                                                                   \
     hipLaunchKernelGGL((k_trace_rays<OCC, CNT>), dim3(G), dim3(kTraceThreads), 0, s, tl, S, rays, count, ray_flags, \
                        cull_mask, sbt_offset, hits, occluded, spill, feed, ctx->d_counters, ctx->d_status,      \
-                       OCC ? ctx->tune.refill_occlusion : ctx->tune.refill_closest, (uint32_t)ctx->tune.feed_chunk)
+                       ctx->tune.refill_rays, (uint32_t)ctx->tune.feed_chunk)
     if (mode == RTG_QUERY_CLOSEST) {
         if (cnt) RTG_LAUNCH_RAYS(false, true);
         else RTG_LAUNCH_RAYS(false, false);

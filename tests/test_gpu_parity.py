@@ -206,6 +206,34 @@ def test_incoherent_full_scene(ctx):
     ds.destroy()
 
 
+def test_trace_rays_refill_threshold_never_changes_results(ctx, monkeypatch):
+    """Ray queries refill a wavefront once RTG_REFILL_RAYS of its lanes are
+    idle (read when a context is created): whole-wave, any-idle and the
+    default threshold give the same hits and occlusion as the oracle."""
+    from rtg.lib import Context
+    s = scenes.incoherent_scene(n_tris=50_000)
+    hs = oh.HostScene(s)
+    rays = scenes.incoherent_rays(1 << 16, seed=0x5EED0103)
+    ref_hits, _ = hs.trace_rays(rays, mode=abi.RTG_QUERY_CLOSEST)
+    ref_occ, _ = hs.trace_rays(rays, mode=abi.RTG_QUERY_OCCLUSION)
+    for refill in ("64", "0", None):
+        if refill is None:
+            monkeypatch.delenv("RTG_REFILL_RAYS", raising=False)
+        else:
+            monkeypatch.setenv("RTG_REFILL_RAYS", refill)
+        c = Context(0)
+        try:
+            ds = DeviceScene(c, s)
+            d = rays_to_device(rays)
+            got = hits_to_numpy(ds.trace_rays(d, len(rays), abi.RTG_QUERY_CLOSEST))
+            assert np.array_equal(got.view(np.uint32), ref_hits.view(np.uint32)), refill
+            occ = ds.trace_rays(d, len(rays), abi.RTG_QUERY_OCCLUSION).cpu().numpy().astype(np.uint32)
+            assert np.array_equal(occ, ref_occ), refill
+            ds.destroy()
+        finally:
+            c.close()
+
+
 def test_bgra_output_and_png(ctx, small, tmp_path):
     """The output step: BGRA8 byte order as the reference's swapchain format,
     equal to the oracle's, and a PNG of the frame that decodes to it."""
