@@ -286,10 +286,30 @@ typedef struct rtg_scene_bindings {
  * NULL when every geometry is opaque. */
 #define RTG_QUERY_CLOSEST 0u
 #define RTG_QUERY_OCCLUSION 1u
+#define RTG_MAX_QUERY_RAYS 0x80000000u /* rays per call (RTG_ERROR_LIMIT beyond) */
 int rtg_trace_rays(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindings* scene,
                    const rtg_ray* rays, uint32_t count, uint32_t mode, uint32_t ray_flags,
                    uint32_t cull_mask, uint32_t sbt_offset, rtg_hit* hits, uint32_t* occluded,
                    void* stream);
+
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): rtg_trace_rays with the same arguments, results and
+ * errors, whose scratch (the rays' feed counter and the spill stacks of the
+ * persistent traversal workgroups) is a file-scope device array of
+ * RTG_STATIC_SCRATCH_BYTES that each GPU holds until the process ends,
+ * instead of memory the context allocates. It calls no hipMalloc, hipFree or
+ * allocating routine and is never timed. Its grid holds at most
+ * RTG_STATIC_QUERY_BLOCKS workgroups of 256 lanes (the spill stacks the array
+ * has room for); the launch shape never changes results. Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (two streams,
+ * two threads, two contexts) race and must be serialised by the caller. */
+#define RTG_STATIC_QUERY_BLOCKS 512u
+#define RTG_STATIC_SCRATCH_BYTES 83886336u /* 256 + 512 * 256 lanes * 640 B of spill stack */
+int rtg_trace_rays_static(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindings* scene,
+                          const rtg_ray* rays, uint32_t count, uint32_t mode, uint32_t ray_flags,
+                          uint32_t cull_mask, uint32_t sbt_offset, rtg_hit* hits, uint32_t* occluded,
+                          void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* The trace step: ≙ vkCmdTraceRaysKHR(W, H, 1) with the reference's        */

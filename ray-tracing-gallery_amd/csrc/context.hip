@@ -100,6 +100,8 @@ extern "C" int rtg_create(rtg_ctx** out, int device) {
     RTG_HIP(hipSetDevice(device));
     hipDeviceProp_t prop;
     RTG_HIP(hipGetDeviceProperties(&prop, device));
+    int sst = static_scratch_init(device);
+    if (sst != RTG_OK) return sst;
     rtg_ctx* ctx = new rtg_ctx();
     ctx->device = device;
     ctx->cu_count = prop.multiProcessorCount;

@@ -521,6 +521,8 @@ __device__ inline bool alpha_clip_keep(const SceneView& S, const rtg_instance* s
 constexpr int kLdsDepth = 16;
 constexpr int kSpillDepth = 80;
 constexpr int kStackMax = kLdsDepth + kSpillDepth;
+// bytes of one lane's spill area
+constexpr size_t kSpillLaneBytes = (size_t)kSpillDepth * 8;
 
 struct Stack {
     uint2* lds;      // lds[k * stride + lane]

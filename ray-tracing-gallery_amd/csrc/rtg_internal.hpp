@@ -186,6 +186,9 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
 size_t spill_bytes(rtg_ctx* ctx, const Tuning& tune);
 // Launches made by the calling thread are not timed (a server's worker).
 void suppress_timing_on_this_thread();
+// Takes the device address of rtg_trace_rays_static's scratch array on
+// `device` (which must be current); once per device, at rtg_create.
+int static_scratch_init(int device);
 // One-kernel refit of the TLAS blob at `tlas` (no registry lookup).
 int launch_refit(rtg_ctx* ctx, uint8_t* tlas, const rtg_instance* instances, uint32_t count, hipStream_t stream);
 }  // namespace rtg

@@ -12,6 +12,7 @@
 // result depends only on its own rays, so queue order never changes results.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "rtg_device.hpp"
@@ -961,7 +962,7 @@ static int trace_grid(rtg_ctx* ctx, const Tuning& tune) {
 static int trace_grid(rtg_ctx* ctx) { return trace_grid(ctx, ctx->tune); }
 
 size_t spill_bytes(rtg_ctx* ctx, const Tuning& tune) {
-    return (size_t)trace_grid(ctx, tune) * kTraceThreads * kSpillDepth * sizeof(uint2);
+    return (size_t)trace_grid(ctx, tune) * kTraceThreads * kSpillLaneBytes;
 }
 
 static int ensure_spill(rtg_ctx* ctx, FrameArena* a, const Tuning& tune) {
@@ -1280,29 +1281,34 @@ extern "C" int rtg_trace_batch(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t
     return trace_jobs(ctx, jobs, n, reinterpret_cast<hipStream_t>(stream), nullptr, true);
 }
 
-extern "C" int rtg_trace_rays(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindings* scene, const rtg_ray* rays,
-                              uint32_t count, uint32_t mode, uint32_t ray_flags, uint32_t cull_mask,
-                              uint32_t sbt_offset, rtg_hit* hits, uint32_t* occluded, void* stream) {
-    clear_error();
-    RTG_REQUIRE(ctx, RTG_ERROR_INVALID_ARGUMENT, "rtg_trace_rays: null context");
+// Arguments of a ray query (rtg_trace_rays, rtg_trace_rays_static).
+static int check_ray_query(rtg_ctx* ctx, rtg_tlas tlas, const rtg_ray* rays, uint32_t count, uint32_t mode,
+                           uint32_t ray_flags, rtg_hit* hits, uint32_t* occluded, const char* what) {
+    RTG_REQUIRE(ctx, RTG_ERROR_INVALID_ARGUMENT, "%s: null context", what);
     auto it = ctx->structures.find(tlas);
     RTG_REQUIRE(it != ctx->structures.end() && it->second.kind == kKindTlas, RTG_ERROR_NOT_FOUND,
-                "rtg_trace_rays: 0x%llx is not a TLAS of this context", (unsigned long long)tlas);
-    RTG_REQUIRE(mode <= RTG_QUERY_OCCLUSION, RTG_ERROR_INVALID_ARGUMENT, "rtg_trace_rays: mode %u", mode);
-    RTG_REQUIRE(count == 0 || rays, RTG_ERROR_INVALID_ARGUMENT, "rtg_trace_rays: null rays");
-    RTG_REQUIRE(mode != RTG_QUERY_CLOSEST || count == 0 || hits, RTG_ERROR_INVALID_ARGUMENT,
-                "rtg_trace_rays: null hits");
+                "%s: 0x%llx is not a TLAS of this context", what, (unsigned long long)tlas);
+    RTG_REQUIRE(mode <= RTG_QUERY_OCCLUSION, RTG_ERROR_INVALID_ARGUMENT, "%s: mode %u", what, mode);
+    RTG_REQUIRE(count == 0 || rays, RTG_ERROR_INVALID_ARGUMENT, "%s: null rays", what);
+    RTG_REQUIRE(mode != RTG_QUERY_CLOSEST || count == 0 || hits, RTG_ERROR_INVALID_ARGUMENT, "%s: null hits", what);
     RTG_REQUIRE(mode != RTG_QUERY_OCCLUSION || count == 0 || occluded, RTG_ERROR_INVALID_ARGUMENT,
-                "rtg_trace_rays: null occluded");
+                "%s: null occluded", what);
+    // the 32-bit feed counter runs up to count + one chunk per wavefront
+    RTG_REQUIRE(count <= RTG_MAX_QUERY_RAYS, RTG_ERROR_LIMIT, "%s: %u rays exceed %u per call", what, count,
+                (uint32_t)RTG_MAX_QUERY_RAYS);
     const uint32_t allowed = RTG_RAY_FLAG_OPAQUE | RTG_RAY_FLAG_NO_OPAQUE | RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT |
                              RTG_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER;
-    RTG_REQUIRE((ray_flags & ~allowed) == 0, RTG_ERROR_INVALID_ARGUMENT, "rtg_trace_rays: unsupported ray flags 0x%x",
+    RTG_REQUIRE((ray_flags & ~allowed) == 0, RTG_ERROR_INVALID_ARGUMENT, "%s: unsupported ray flags 0x%x", what,
                 ray_flags & ~allowed);
-    if (count == 0) return RTG_OK;
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    RTG_HIP(hipSetDevice(ctx->device));
-    FrameArena* a = arena(ctx, s);
-    if (ensure_spill(ctx, a, ctx->tune) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
+    return RTG_OK;
+}
+
+// Clears the feed counter and launches the query kernel on `G` persistent
+// blocks whose spill stacks start at `spill` (G * kTraceThreads lanes).
+static void launch_ray_query(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindings* scene, const rtg_ray* rays,
+                             uint32_t count, uint32_t mode, uint32_t ray_flags, uint32_t cull_mask,
+                             uint32_t sbt_offset, rtg_hit* hits, uint32_t* occluded, uint2* spill, uint32_t* feed,
+                             unsigned G, bool timed, hipStream_t s) {
     SceneView S;
     S.models = scene ? reinterpret_cast<const rtg_model_info*>(scene->model_info) : nullptr;
     S.textures = scene ? reinterpret_cast<const rtg_texture*>(scene->textures) : nullptr;
@@ -1310,14 +1316,11 @@ extern "C" int rtg_trace_rays(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindi
     S.model_count = scene ? scene->model_count : 0;
     S.texture_count = scene ? scene->texture_count : 0;
     const bool cnt = ctx->counters_enabled;
-    const unsigned G = (unsigned)std::min<uint64_t>((count + kTraceThreads - 1) / kTraceThreads, trace_grid(ctx));
     uint8_t* tl = reinterpret_cast<uint8_t*>(tlas);
-    uint2* spill = reinterpret_cast<uint2*>(a->spill.ptr);
-    uint32_t* feed = reinterpret_cast<uint32_t*>(a->feed.ptr);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, feed, 1u);
     // the primary ray count of a ray query is the query size
     if (cnt) hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, ctx->d_counters, (uint64_t)count);
-    LaunchTimer t(ctx, mode == RTG_QUERY_CLOSEST ? RTG_KERNEL_TRACE_CLOSEST : RTG_KERNEL_TRACE_OCCLUSION, s);
+    LaunchTimer t(ctx, mode == RTG_QUERY_CLOSEST ? RTG_KERNEL_TRACE_CLOSEST : RTG_KERNEL_TRACE_OCCLUSION, s, timed);
     cull_mask &= 0xffu;
 #define RTG_LAUNCH_RAYS(OCC, CNT)                                                                                   \
     hipLaunchKernelGGL((k_trace_rays<OCC, CNT>), dim3(G), dim3(kTraceThreads), 0, s, tl, S, rays, count, ray_flags, \
@@ -1331,6 +1334,88 @@ extern "C" int rtg_trace_rays(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindi
         else RTG_LAUNCH_RAYS(true, false);
     }
 #undef RTG_LAUNCH_RAYS
+}
+
+extern "C" int rtg_trace_rays(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindings* scene, const rtg_ray* rays,
+                              uint32_t count, uint32_t mode, uint32_t ray_flags, uint32_t cull_mask,
+                              uint32_t sbt_offset, rtg_hit* hits, uint32_t* occluded, void* stream) {
+    clear_error();
+    int st = check_ray_query(ctx, tlas, rays, count, mode, ray_flags, hits, occluded, "rtg_trace_rays");
+    if (st != RTG_OK) return st;
+    if (count == 0) return RTG_OK;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    RTG_HIP(hipSetDevice(ctx->device));
+    FrameArena* a = arena(ctx, s);
+    if (ensure_spill(ctx, a, ctx->tune) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
+    const unsigned G = (unsigned)std::min<uint64_t>((count + kTraceThreads - 1) / kTraceThreads, trace_grid(ctx));
+    launch_ray_query(ctx, tlas, scene, rays, count, mode, ray_flags, cull_mask, sbt_offset, hits, occluded,
+                     reinterpret_cast<uint2*>(a->spill.ptr), reinterpret_cast<uint32_t*>(a->feed.ptr), G, true, s);
+    RTG_HIP(hipGetLastError());
+    return RTG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// rtg_trace_rays_static: the ray query with its scratch (the feed counter and
+// the spill stacks of kStaticBlocks persistent workgroups) in a file-scope
+// device array instead of the context's arenas, for callers that cannot
+// allocate. Every GPU holds its own copy of `scratch` while the process
+// lives; its address on a device is taken once, at rtg_create, with that
+// device current, and only that address is handed to kernels. Calls that
+// overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+namespace rtg {
+constexpr unsigned kStaticBlocks = RTG_STATIC_QUERY_BLOCKS;
+constexpr size_t kStaticFeedBytes = 256;
+constexpr size_t kStaticScratchBytes = kStaticFeedBytes + (size_t)kStaticBlocks * kTraceThreads * kSpillLaneBytes;
+static_assert(kStaticScratchBytes == RTG_STATIC_SCRATCH_BYTES, "rtg.h states the static scratch size");
+
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+int static_scratch_init(int device) {
+    RTG_REQUIRE(device >= 0 && device < kMaxDevices, RTG_ERROR_LIMIT, "device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return RTG_OK;
+    int cur = -1;
+    RTG_HIP(hipGetDevice(&cur));
+    RTG_REQUIRE(cur == device, RTG_ERROR_STATE, "static scratch of device %d set up with device %d current", device,
+                cur);
+    void* p = nullptr;
+    RTG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    RTG_REQUIRE(p != nullptr, RTG_ERROR_DEVICE, "hipGetSymbolAddress(scratch) gave a null address on device %d",
+                device);
+    g_scratch_ptr[device] = p;
+    return RTG_OK;
+}
+
+static void* static_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+}  // namespace rtg
+
+extern "C" int rtg_trace_rays_static(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindings* scene,
+                                     const rtg_ray* rays, uint32_t count, uint32_t mode, uint32_t ray_flags,
+                                     uint32_t cull_mask, uint32_t sbt_offset, rtg_hit* hits, uint32_t* occluded,
+                                     void* stream) {
+    clear_error();
+    int st = check_ray_query(ctx, tlas, rays, count, mode, ray_flags, hits, occluded, "rtg_trace_rays_static");
+    if (st != RTG_OK) return st;
+    if (count == 0) return RTG_OK;
+    RTG_HIP(hipSetDevice(ctx->device));
+    uint8_t* base = static_cast<uint8_t*>(static_scratch(ctx->device));
+    RTG_REQUIRE(base != nullptr, RTG_ERROR_STATE, "rtg_trace_rays_static: no static scratch on device %d",
+                ctx->device);
+    // the grid is capped by the spill stacks the array holds; the launch
+    // shape never changes results
+    const unsigned G = (unsigned)std::min<uint64_t>(
+        std::min<uint64_t>((count + kTraceThreads - 1) / kTraceThreads, trace_grid(ctx)), kStaticBlocks);
+    launch_ray_query(ctx, tlas, scene, rays, count, mode, ray_flags, cull_mask, sbt_offset, hits, occluded,
+                     reinterpret_cast<uint2*>(base + kStaticFeedBytes), reinterpret_cast<uint32_t*>(base), G, false,
+                     reinterpret_cast<hipStream_t>(stream));
     RTG_HIP(hipGetLastError());
     return RTG_OK;
 }

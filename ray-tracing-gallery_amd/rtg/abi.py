@@ -338,4 +338,11 @@ class ServerStats(C.Structure):
         return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
+RTG_OK = 0
+RTG_ERROR_INVALID_ARGUMENT = -1
+RTG_ERROR_OUT_OF_MEMORY = -2
+RTG_ERROR_DEVICE = -3
+RTG_ERROR_LIMIT = -4
+RTG_ERROR_NOT_FOUND = -5
 RTG_ERROR_BUDGET = -6
+RTG_ERROR_STATE = -7

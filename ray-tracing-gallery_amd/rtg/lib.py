@@ -16,7 +16,7 @@ EXPORTS = [
     "rtg_create", "rtg_destroy", "rtg_last_error", "rtg_abi_version",
     "rtg_blas_query_sizes", "rtg_blas_build", "rtg_blas_destroy",
     "rtg_tlas_query_sizes", "rtg_tlas_build", "rtg_tlas_refit", "rtg_tlas_clone", "rtg_tlas_destroy",
-    "rtg_as_get_info", "rtg_trace_rays", "rtg_trace_image", "rtg_trace_batch",
+    "rtg_as_get_info", "rtg_trace_rays", "rtg_trace_rays_static", "rtg_trace_image", "rtg_trace_batch",
     "rtg_set_counters", "rtg_get_counters", "rtg_get_status", "rtg_set_timing", "rtg_get_kernel_time",
     "rtg_server_create", "rtg_server_add_scene", "rtg_server_submit", "rtg_server_poll", "rtg_server_replay",
     "rtg_server_clock", "rtg_server_get_stats", "rtg_server_destroy",
@@ -56,6 +56,7 @@ def load():
     L.rtg_tlas_destroy.argtypes = [vp, u64]
     L.rtg_as_get_info.argtypes = [vp, u64, C.POINTER(abi.AsInfo)]
     L.rtg_trace_rays.argtypes = [vp, u64, C.POINTER(abi.SceneBindings), u64, u32, u32, u32, u32, u32, u64, u64, vp]
+    L.rtg_trace_rays_static.argtypes = L.rtg_trace_rays.argtypes
     L.rtg_trace_image.argtypes = [vp, C.POINTER(abi.ImageJob), vp]
     L.rtg_trace_batch.argtypes = [vp, vp, u32, vp]
     L.rtg_set_counters.argtypes = [vp, i32]
