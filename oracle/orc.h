@@ -93,6 +93,33 @@ void orc_watertight(const float o[3], const float d[3], const float v0[3], const
                     float* v);
 uint32_t orc_morton3(float cx, float cy, float cz, const float lo[3], const float hi[3]);
 
+/* The stage functions the per-pixel trace uses (exported for the tests that
+ * pin each one). Matrices are the 3x4 row-major object-to-world transform of
+ * the instance; points and vectors are 3 floats. */
+/* ray_generation's camera ray for pixel (px, py) + (jx, jy) (0.5 = centre). */
+void orc_camera_ray(const rtg_uniforms* uniforms, uint32_t width, uint32_t height, uint32_t px, uint32_t py,
+                    float jx, float jy, float origin[3], float dir[3]);
+/* primary_ray_miss: sun disk or sky colour. */
+void orc_primary_miss(const float dir[3], const float sun_dir[3], float sun_radius, float out[3]);
+/* The shadow-terminator-fixed hit point in model space (pos, nrm: the
+ * triangle's three vertex positions and normals; bc: barycentrics). */
+void orc_terminator_fix(const float pos[9], const float nrm[9], const float bc[3], float out[3]);
+/* The two animated blue-noise numbers of shadow ray i of pixel (px, py). */
+void orc_blue_noise(const rtg_texture* blue_noise, const float srgb_lut[256], uint32_t px, uint32_t py,
+                    uint32_t i, uint32_t frame, float out[2]);
+/* The world-space shading normal (map_rgb: the normal map's sample, or NULL). */
+void orc_shading_normal(const float o2w[12], const float nrm[3], const float pos[9], const float uv[6],
+                        const float* map_rgb, float out[3]);
+/* closest_hit_textured's colour: BRDF * sun_factor + 0.1 * base. */
+void orc_textured_colour(const float normal[3], const float ray_dir[3], const float sun_dir[3], float roughness,
+                         const float base[3], float metallic, float sun_factor, float out[3]);
+/* closest_hit_mirror and closest_hit_portal continuation rays. */
+void orc_mirror(const float o2w[12], const float normal[3], const float origin[3], const float dir[3], float t,
+                float new_origin[3], float new_dir[3]);
+void orc_portal(const float origin[3], const float dir[3], float t, float new_origin[3], float new_dir[3]);
+/* Heatmap mode: heatmap_temperature(cycles / 1e6) + rgb * 1e-6 per pixel. */
+void orc_heatmap_pixels(const uint32_t* cycles, const float* rgb, uint32_t n, float* out);
+
 #ifdef __cplusplus
 }
 #endif

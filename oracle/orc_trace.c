@@ -724,6 +724,194 @@ static v2 load_v2(uint64_t base, uint32_t i) {
     return r;
 }
 
+/* ---------------------------------------------------------------- */
+/* The stage functions of the trace step. Each is used by the per-pixel */
+/* path below and exported (orc_*) for the tests that pin it.           */
+/* ---------------------------------------------------------------- */
+
+/* ray_generation's camera ray (shaders/ray-tracing/src/lib.rs:126-142):
+ * pixel point (px + jx, py + jy) / launch size * 2 - 1 = render coordinates
+ * (x, y); origin = view_inverse * (0, 0, 0, 1); direction = view_inverse *
+ * (normalize((proj_inverse * (x, y, 1, 1)).xyz), 0). No perspective divide:
+ * the w of proj_inverse * (x, y, 1, 1) is dropped. */
+static void camera_ray(const rtg_uniforms* U, uint32_t width, uint32_t height, uint32_t px, uint32_t py, float jx,
+                       float jy, v3* origin, v3* dir) {
+    float pcx = (float)px + jx, pcy = (float)py + jy;
+    float tcx = pcx / (float)width, tcy = pcy / (float)height;
+    float rcx = tcx * 2.0f - 1.0f, rcy = tcy * 2.0f - 1.0f;
+    float zero_one[4] = {0.0f, 0.0f, 0.0f, 1.0f}, tmp[4];
+    mat4_mul_vec4(U->view_inverse, zero_one, tmp);
+    *origin = V3(tmp[0], tmp[1], tmp[2]);
+    float tv[4] = {rcx, rcy, 1.0f, 1.0f};
+    mat4_mul_vec4(U->proj_inverse, tv, tmp);
+    v3 ld = v3_normalize(V3(tmp[0], tmp[1], tmp[2]));
+    float dv[4] = {ld.x, ld.y, ld.z, 0.0f};
+    mat4_mul_vec4(U->view_inverse, dv, tmp);
+    *dir = V3(tmp[0], tmp[1], tmp[2]);
+}
+void orc_camera_ray(const rtg_uniforms* U, uint32_t width, uint32_t height, uint32_t px, uint32_t py, float jx,
+                    float jy, float origin[3], float dir[3]) {
+    v3 o, d;
+    camera_ray(U, width, height, px, py, jx, jy, &o, &d);
+    origin[0] = o.x; origin[1] = o.y; origin[2] = o.z;
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+}
+
+/* primary_ray_miss (lib.rs:38-51): the sun disk is white where the ray lies
+ * within sun_radius of sun_dir (dot > cos(sun_radius)), the sky is
+ * (0, 0, 0.05). cos_sun = cos(sun_radius). */
+static v3 primary_miss(v3 dir, v3 sun, float cos_sun) {
+    return v3_dot(dir, sun) > cos_sun ? V3(1.0f, 1.0f, 1.0f) : V3(0.0f, 0.0f, 0.05f);
+}
+void orc_primary_miss(const float dir[3], const float sun_dir[3], float sun_radius, float out[3]) {
+    v3 c = primary_miss(V3(dir[0], dir[1], dir[2]), V3(sun_dir[0], sun_dir[1], sun_dir[2]), orc_cosf(sun_radius));
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+}
+
+/* get_shadow_terminator_fix_shadow_origin (closest_hit_textured.glsl:13-39;
+ * Ray Tracing Gems II ch. 4.3), model space: the hit point P = interpolate(
+ * positions, bc) plus the interpolation of each vertex's offset
+ * (P - P_i) - min(0, (P - P_i) . n_i) n_i. */
+static v3 terminator_fix(const v3 P[3], const v3 N[3], v3 bc) {
+    v3 pos = v3_interp(P[0], P[1], P[2], bc);
+    v3 off[3];
+    for (int k = 0; k < 3; ++k) {
+        v3 vtp = v3_sub(pos, P[k]);
+        float dp = fminf(0.0f, v3_dot(vtp, N[k]));
+        off[k] = v3_sub(vtp, v3_scale(N[k], dp));
+    }
+    return v3_add(pos, v3_interp(off[0], off[1], off[2], bc));
+}
+void orc_terminator_fix(const float pos[9], const float nrm[9], const float bc[3], float out[3]) {
+    v3 P[3], N[3];
+    for (int k = 0; k < 3; ++k) {
+        P[k] = V3(pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]);
+        N[k] = V3(nrm[3 * k], nrm[3 * k + 1], nrm[3 * k + 2]);
+    }
+    v3 r = terminator_fix(P, N, V3(bc[0], bc[1], bc[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* sample_blue_noise (closest_hit_textured.glsl:96-111) and
+ * animated_blue_noise (:113-120): the two numbers of shadow ray i of pixel
+ * (px, py): the .r of the 64x64 mask at (px, py) + 2i (13, 41) and at
+ * (px, py) + (2i + 1)(13, 41), over 64, each plus (frame % 32) times the
+ * golden ratio's fraction, modulo 1. `bn` NULL samples as (0, 0, 0, 1). */
+static void blue_noise_pair(const rtg_texture* bn, const float* lut, uint32_t px, uint32_t py, uint32_t i,
+                            uint32_t frame, float* rx, float* ry) {
+    static const rtg_texture none = {0, 0, 0, 0, 0};
+    if (!bn) bn = &none;
+    uint32_t o1x = i * 2u * 13u, o1y = i * 2u * 41u;
+    uint32_t o2x = (i * 2u + 1u) * 13u, o2y = (i * 2u + 1u) * 41u;
+    float t1[4], t2[4];
+    sample_tex(bn, lut, (float)(px + o1x) / 64.0f, (float)(py + o1y) / 64.0f, t1);
+    sample_tex(bn, lut, (float)(px + o2x) / 64.0f, (float)(py + o2y) / 64.0f, t2);
+    float anim = (float)(frame % 32u) * 0.618033988749f;
+    *rx = fractf(t1[0] + anim);
+    *ry = fractf(t2[0] + anim);
+}
+void orc_blue_noise(const rtg_texture* bn, const float lut[256], uint32_t px, uint32_t py, uint32_t i,
+                    uint32_t frame, float out[2]) {
+    blue_noise_pair(bn, lut, px, py, i, frame, &out[0], &out[1]);
+}
+
+/* calculate_normal (closest_hit_textured.glsl:141-157): without a normal map
+ * the interpolated normal rotated into world space by
+ * mat3(gl_WorldToObject3x4EXT) (:69-72) and normalised; with one, the map's
+ * rgb * 2 - 1, normalised, through the cotangent frame of the triangle
+ * (compute_cotangent_frame :123-139) first. map_rgb NULL: no normal map. */
+static v3 shading_normal(const float o2w[12], v3 nrm, const v3 P[3], const v2 UV[3], const float* map_rgb) {
+    float w2o[12];
+    m34_inverse(o2w, w2o);
+    if (!map_rgb) return v3_normalize(m34_transpose_dir(w2o, nrm));
+    v3 mapn = V3(map_rgb[0] * 2.0f - 1.0f, map_rgb[1] * 2.0f - 1.0f, map_rgb[2] * 2.0f - 1.0f);
+    v3 dp1 = v3_sub(P[1], P[0]), dp2 = v3_sub(P[2], P[0]);
+    float du1x = UV[1].x - UV[0].x, du1y = UV[1].y - UV[0].y, du2x = UV[2].x - UV[0].x, du2y = UV[2].y - UV[0].y;
+    v3 dp2perp = v3_cross(dp2, nrm), dp1perp = v3_cross(nrm, dp1);
+    v3 T = v3_add(v3_scale(dp2perp, du1x), v3_scale(dp1perp, du2x));
+    v3 B = v3_add(v3_scale(dp2perp, du1y), v3_scale(dp1perp, du2y));
+    float invmax = 1.0f / sqrtf(fmaxf(v3_dot(T, T), v3_dot(B, B)));
+    v3 tt = v3_scale(T, invmax), bb = v3_scale(B, invmax);
+    v3 m = v3_normalize(mapn);
+    v3 local = v3_add(v3_add(v3_scale(tt, m.x), v3_scale(bb, m.y)), v3_scale(nrm, m.z));
+    return v3_normalize(m34_transpose_dir(w2o, local));
+}
+void orc_shading_normal(const float o2w[12], const float nrm[3], const float pos[9], const float uv[6],
+                        const float* map_rgb, float out[3]) {
+    v3 P[3];
+    v2 UV[3];
+    for (int k = 0; k < 3; ++k) {
+        P[k] = V3(pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]);
+        UV[k].x = uv[2 * k];
+        UV[k].y = uv[2 * k + 1];
+    }
+    v3 r = shading_normal(o2w, V3(nrm[0], nrm[1], nrm[2]), P, UV, map_rgb);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* closest_hit_textured's colour (closest_hit_textured.glsl:203-225): the
+ * BRDF towards the centre sun direction with light intensity sun_factor (the
+ * mean visibility of the shadow rays) and reflectance 0.5, plus the ambient
+ * term 0.1 * base colour. */
+static v3 textured_colour(v3 normal, v3 ray_dir, v3 sun, float roughness, v3 base, float metallic,
+                          float sun_factor) {
+    v3 col = brdf(normal, v3_neg(ray_dir), sun, roughness, base, metallic, 0.5f, sun_factor);
+    v3 amb = V3(0.1f * base.x, 0.1f * base.y, 0.1f * base.z);
+    return v3_add(col, amb);
+}
+void orc_textured_colour(const float normal[3], const float ray_dir[3], const float sun_dir[3], float roughness,
+                         const float base[3], float metallic, float sun_factor, float out[3]) {
+    v3 r = textured_colour(V3(normal[0], normal[1], normal[2]), V3(ray_dir[0], ray_dir[1], ray_dir[2]),
+                           V3(sun_dir[0], sun_dir[1], sun_dir[2]), roughness, V3(base[0], base[1], base[2]),
+                           metallic, sun_factor);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* closest_hit_mirror (closest_hit_mirror.glsl:11-29): the continuation ray
+ * leaves the hit point o + d t along reflect(d, n), n = the interpolated
+ * normal rotated by mat3(gl_WorldToObject3x4EXT) and normalised;
+ * reflect(d, n) = d - 2 (n . d) n. */
+static void mirror_stage(const float o2w[12], v3 n_interp, v3 o, v3 d, float t, v3* new_o, v3* new_d) {
+    float w2o[12];
+    m34_inverse(o2w, w2o);
+    v3 n = v3_normalize(m34_transpose_dir(w2o, n_interp));
+    float k = 2.0f * v3_dot(n, d);
+    *new_d = v3_sub(d, v3_scale(n, k));
+    *new_o = v3_add(o, v3_scale(d, t));
+}
+void orc_mirror(const float o2w[12], const float normal[3], const float origin[3], const float dir[3], float t,
+                float new_origin[3], float new_dir[3]) {
+    v3 no, nd;
+    mirror_stage(o2w, V3(normal[0], normal[1], normal[2]), V3(origin[0], origin[1], origin[2]),
+                 V3(dir[0], dir[1], dir[2]), t, &no, &nd);
+    new_origin[0] = no.x; new_origin[1] = no.y; new_origin[2] = no.z;
+    new_dir[0] = nd.x; new_dir[1] = nd.y; new_dir[2] = nd.z;
+}
+
+/* closest_hit_portal (shaders/ray-tracing/src/lib.rs:300-312): the ray
+ * continues in its direction from the hit point o + d t moved by (0, 5, 0). */
+static void portal_stage(v3 o, v3 d, float t, v3* new_o, v3* new_d) {
+    *new_d = d;
+    *new_o = v3_add(v3_add(o, v3_scale(d, t)), V3(0.0f, 5.0f, 0.0f));
+}
+void orc_portal(const float origin[3], const float dir[3], float t, float new_origin[3], float new_dir[3]) {
+    v3 no, nd;
+    portal_stage(V3(origin[0], origin[1], origin[2]), V3(dir[0], dir[1], dir[2]), t, &no, &nd);
+    new_origin[0] = no.x; new_origin[1] = no.y; new_origin[2] = no.z;
+    new_dir[0] = nd.x; new_dir[1] = nd.y; new_dir[2] = nd.z;
+}
+
+/* The heatmap branch of ray_generation (lib.rs:174-186): colour =
+ * heatmap_temperature(cycles / 1e6) + colour * 1e-6, per pixel, for the
+ * traversal cycle counts a device measured (n pixels, rgb in and out). */
+void orc_heatmap_pixels(const uint32_t* cycles, const float* rgb, uint32_t n, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        float hm[3];
+        orc_heatmap_temperature((float)cycles[i] / 1000000.0f, hm);
+        for (int c = 0; c < 3; ++c) out[3 * i + c] = hm[c] + rgb[3 * i + c] * 0.000001f;
+    }
+}
+
 /* closest_hit_textured main (closest_hit_textured.glsl:174-226). Returns the
  * final colour given the hit; traces its own shadow rays. */
 static v3 shade_textured(qctx* q, const rtg_image_job* job, uint32_t px, uint32_t py, uint32_t frame,
@@ -735,45 +923,30 @@ static v3 shade_textured(qctx* q, const rtg_image_job* job, uint32_t px, uint32_
     const rtg_geometry_info* gi = (const rtg_geometry_info*)(uintptr_t)mi->geometry_info_address + h->geom;
     const uint32_t* idx = (const uint32_t*)(uintptr_t)gi->index_buffer_address + 3ull * h->prim;
     uint32_t i0 = idx[0], i1 = idx[1], i2 = idx[2];
-    v3 pa = load_v3(mi->position_buffer_address, i0), pb = load_v3(mi->position_buffer_address, i1),
-       pc = load_v3(mi->position_buffer_address, i2);
-    v3 na = load_v3(mi->normal_buffer_address, i0), nb = load_v3(mi->normal_buffer_address, i1),
-       nc = load_v3(mi->normal_buffer_address, i2);
-    v2 ua = load_v2(mi->uv_buffer_address, i0), ub = load_v2(mi->uv_buffer_address, i1),
-       uc = load_v2(mi->uv_buffer_address, i2);
+    v3 P[3] = {load_v3(mi->position_buffer_address, i0), load_v3(mi->position_buffer_address, i1),
+               load_v3(mi->position_buffer_address, i2)};
+    v3 N[3] = {load_v3(mi->normal_buffer_address, i0), load_v3(mi->normal_buffer_address, i1),
+               load_v3(mi->normal_buffer_address, i2)};
+    v2 UV[3] = {load_v2(mi->uv_buffer_address, i0), load_v2(mi->uv_buffer_address, i1),
+                load_v2(mi->uv_buffer_address, i2)};
     v3 bc = V3((1.0f - h->u) - h->v, h->u, h->v);
-    v3 pos = v3_interp(pa, pb, pc, bc);
-    v3 nrm = v3_interp(na, nb, nc, bc);
-    v2 uv = v2_interp(ua, ub, uc, bc);
-
-    /* get_shadow_terminator_fix_shadow_origin (:13-39) */
-    v3 off[3];
-    v3 P[3] = {pa, pb, pc}, N[3] = {na, nb, nc};
-    for (int k = 0; k < 3; ++k) {
-        v3 vtp = v3_sub(pos, P[k]);
-        float dp = fminf(0.0f, v3_dot(vtp, N[k]));
-        off[k] = v3_sub(vtp, v3_scale(N[k], dp));
-    }
-    v3 fixed = v3_add(pos, v3_interp(off[0], off[1], off[2], bc));
+    v3 nrm = v3_interp(N[0], N[1], N[2], bc);
+    v2 uv = v2_interp(UV[0], UV[1], UV[2], bc);
     const float* o2w = (const float*)src->transform;
-    v3 shadow_origin = m34_point(o2w, fixed);
+    v3 shadow_origin = m34_point(o2w, terminator_fix(P, N, bc));
 
     const rtg_uniforms* U = &job->uniforms;
     v3 sun = V3(U->sun_dir[0], U->sun_dir[1], U->sun_dir[2]);
+    const rtg_texture* bn =
+        U->blue_noise_texture_index < q->texture_count ? &q->textures[U->blue_noise_texture_index] : NULL;
     uint32_t nsh = job->opts.shadow_rays;
     float vis = 0.0f;
     uint32_t saved_flags = q->ray_flags, saved_sbt = q->sbt_offset;
     q->ray_flags = RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RTG_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER;
     q->sbt_offset = 1;
     for (uint32_t i = 0; i < nsh; ++i) {
-        /* sample_blue_noise (:99-111) + animated_blue_noise (:116-120) */
-        uint32_t o1x = i * 2u * 13u, o1y = i * 2u * 41u;
-        uint32_t o2x = (i * 2u + 1u) * 13u, o2y = (i * 2u + 1u) * 41u;
-        float t1[4], t2[4];
-        sample_index(q, U->blue_noise_texture_index, (float)(px + o1x) / 64.0f, (float)(py + o1y) / 64.0f, t1);
-        sample_index(q, U->blue_noise_texture_index, (float)(px + o2x) / 64.0f, (float)(py + o2y) / 64.0f, t2);
-        float anim = (float)(frame % 32u) * 0.618033988749f;
-        float rx = fractf(t1[0] + anim), ry = fractf(t2[0] + anim);
+        float rx, ry;
+        blue_noise_pair(bn, q->lut, px, py, i, frame, &rx, &ry);
         v3 sd = sample_dir_light(rx, ry, sun, U->sun_radius);
         hitrec sh;
         q->cnt.shadow_rays++;
@@ -791,50 +964,28 @@ static v3 shade_textured(qctx* q, const rtg_image_job* job, uint32_t px, uint32_
     v3 base = V3(dt[0], dt[1], dt[2]);
     float metallic = mr[2], roughness = mr[1];
 
-    /* calculate_normal (:141-157) */
-    float w2o[12];
-    m34_inverse(o2w, w2o);
-    v3 normal;
-    if (gi->normal_map_image_index < 0) {
-        normal = v3_normalize(m34_transpose_dir(w2o, nrm));
-    } else {
-        float mt[4];
+    float mt[4];
+    const float* map_rgb = NULL;
+    if (gi->normal_map_image_index >= 0) {
         sample_index(q, (uint32_t)gi->normal_map_image_index, uv.x, uv.y, mt);
-        v3 mapn = V3(mt[0] * 2.0f - 1.0f, mt[1] * 2.0f - 1.0f, mt[2] * 2.0f - 1.0f);
-        /* compute_cotangent_frame (:123-139) */
-        v3 dp1 = v3_sub(pb, pa), dp2 = v3_sub(pc, pa);
-        float du1x = ub.x - ua.x, du1y = ub.y - ua.y, du2x = uc.x - ua.x, du2y = uc.y - ua.y;
-        v3 dp2perp = v3_cross(dp2, nrm), dp1perp = v3_cross(nrm, dp1);
-        v3 T = v3_add(v3_scale(dp2perp, du1x), v3_scale(dp1perp, du2x));
-        v3 B = v3_add(v3_scale(dp2perp, du1y), v3_scale(dp1perp, du2y));
-        float invmax = 1.0f / sqrtf(fmaxf(v3_dot(T, T), v3_dot(B, B)));
-        v3 tt = v3_scale(T, invmax), bb = v3_scale(B, invmax);
-        v3 m = v3_normalize(mapn);
-        v3 local = v3_add(v3_add(v3_scale(tt, m.x), v3_scale(bb, m.y)), v3_scale(nrm, m.z));
-        normal = v3_normalize(m34_transpose_dir(w2o, local));
+        map_rgb = mt;
     }
-    v3 col = brdf(normal, v3_neg(ray_dir), sun, roughness, base, metallic, 0.5f, sun_factor);
-    v3 amb = V3(0.1f * base.x, 0.1f * base.y, 0.1f * base.z);
-    return v3_add(col, amb);
+    v3 normal = shading_normal(o2w, nrm, P, UV, map_rgb);
+    return textured_colour(normal, ray_dir, sun, roughness, base, metallic, sun_factor);
 }
 
-/* closest_hit_mirror (closest_hit_mirror.glsl:11-29): new direction. */
-static v3 mirror_dir(qctx* q, const hitrec* h, v3 d) {
+/* The normal of the hit triangle interpolated in model space (the mirror
+ * stage's interpolate(read_vec3_triangle(normals), bc),
+ * closest_hit_mirror.glsl:21). */
+static v3 hit_normal(const qctx* q, const hitrec* h) {
     const rtg_instance* src = &q->tsrc[h->inst];
-    uint32_t custom = src->custom_index_and_mask & 0xffffffu;
-    if (custom >= q->model_count) return V3(0.0f, 0.0f, 0.0f);
-    const rtg_model_info* mi = &q->models[custom];
+    const rtg_model_info* mi = &q->models[src->custom_index_and_mask & 0xffffffu];
     const rtg_geometry_info* gi = (const rtg_geometry_info*)(uintptr_t)mi->geometry_info_address + h->geom;
     const uint32_t* idx = (const uint32_t*)(uintptr_t)gi->index_buffer_address + 3ull * h->prim;
     v3 na = load_v3(mi->normal_buffer_address, idx[0]), nb = load_v3(mi->normal_buffer_address, idx[1]),
        nc = load_v3(mi->normal_buffer_address, idx[2]);
     v3 bc = V3((1.0f - h->u) - h->v, h->u, h->v);
-    v3 n_i = v3_interp(na, nb, nc, bc);
-    float w2o[12];
-    m34_inverse((const float*)src->transform, w2o);
-    v3 n = v3_normalize(m34_transpose_dir(w2o, n_i));
-    float k = 2.0f * v3_dot(n, d);
-    return v3_sub(d, v3_scale(n, k));
+    return v3_interp(na, nb, nc, bc);
 }
 
 /* One sample of one pixel: ray_generation (lib.rs:94-191). */
@@ -848,18 +999,8 @@ static v3 trace_sample(qctx* q, const image_ctx* ic, uint32_t px, uint32_t py, u
         jy = fractf(0.5f + (float)s * 0.569840296f);
     }
     uint32_t frame = U->frame_index * spp + s;
-    float pcx = (float)px + jx, pcy = (float)py + jy;
-    float tcx = pcx / (float)job->width, tcy = pcy / (float)job->height;
-    float rcx = tcx * 2.0f - 1.0f, rcy = tcy * 2.0f - 1.0f;
-    float zero_one[4] = {0.0f, 0.0f, 0.0f, 1.0f}, tmp[4];
-    mat4_mul_vec4(U->view_inverse, zero_one, tmp);
-    v3 origin = V3(tmp[0], tmp[1], tmp[2]);
-    float tv[4] = {rcx, rcy, 1.0f, 1.0f};
-    mat4_mul_vec4(U->proj_inverse, tv, tmp);
-    v3 ld = v3_normalize(V3(tmp[0], tmp[1], tmp[2]));
-    float dv[4] = {ld.x, ld.y, ld.z, 0.0f};
-    mat4_mul_vec4(U->view_inverse, dv, tmp);
-    v3 dir = V3(tmp[0], tmp[1], tmp[2]);
+    v3 origin, dir;
+    camera_ray(U, job->width, job->height, px, py, jx, jy, &origin, &dir);
     v3 colour = V3(0.0f, 0.0f, 0.0f);
     for (uint32_t tr = 0; tr < job->opts.max_traces; ++tr) {
         colour = V3(0.0f, 0.0f, 0.0f);
@@ -879,20 +1020,18 @@ static v3 trace_sample(qctx* q, const image_ctx* ic, uint32_t px, uint32_t py, u
             primary->v = h.v;
         }
         if (f <= 0) {
-            /* primary_ray_miss (lib.rs:40-51) */
             v3 sun = V3(U->sun_dir[0], U->sun_dir[1], U->sun_dir[2]);
-            colour = v3_dot(dir, sun) > ic->cos_sun ? V3(1.0f, 1.0f, 1.0f) : V3(0.0f, 0.0f, 0.05f);
+            colour = primary_miss(dir, sun, ic->cos_sun);
         } else {
-            uint32_t group = q->tsrc[h.inst].sbt_offset_and_flags & 0xffffffu;
+            const rtg_instance* src = &q->tsrc[h.inst];
+            uint32_t group = src->sbt_offset_and_flags & 0xffffffu;
+            int has_model = (src->custom_index_and_mask & 0xffffffu) < q->model_count;
             if (group == RTG_HIT_TEXTURED) {
                 colour = shade_textured(q, job, px, py, frame, &h, dir);
-            } else if (group == RTG_HIT_MIRROR) {
-                nd = mirror_dir(q, &h, dir);
-                no = v3_add(origin, v3_scale(dir, h.t));
+            } else if (group == RTG_HIT_MIRROR && has_model) {
+                mirror_stage((const float*)src->transform, hit_normal(q, &h), origin, dir, h.t, &no, &nd);
             } else if (group == RTG_HIT_PORTAL) {
-                /* closest_hit_portal (lib.rs:300-312) */
-                nd = dir;
-                no = v3_add(v3_add(origin, v3_scale(dir, h.t)), V3(0.0f, 5.0f, 0.0f));
+                portal_stage(origin, dir, h.t, &no, &nd);
             }
         }
         if (nd.x == 0.0f && nd.y == 0.0f && nd.z == 0.0f) break;

@@ -367,6 +367,21 @@ def proj_inverse_reversed_z(fov_y, aspect, near=0.1):
     return m.astype(np.float32)
 
 
+def fps_view_inverse(eye, pitch, yaw):
+    """Camera-to-world matrix of the reference's first-person camera
+    (FirstPersonCamera::as_view_matrix, src/main.rs:1205-1231): the view
+    matrix's rows are the right, up and back axes, so its inverse has them as
+    columns, with the eye as translation. pitch > 0 looks up; yaw = pi looks
+    along +z."""
+    cy, sy, cp, sp = math.cos(yaw), math.sin(yaw), math.cos(pitch), math.sin(pitch)
+    m = np.eye(4)
+    m[:3, 0] = (cy, 0.0, -sy)
+    m[:3, 1] = (sy * sp, cp, cy * sp)
+    m[:3, 2] = (sy * cp, -sp, cp * cy)
+    m[:3, 3] = eye
+    return m.astype(np.float32)
+
+
 def sun_direction(pitch, yaw):
     """Sun::as_normal (src/main.rs:1189-1197)."""
     return np.array([math.cos(pitch) * math.sin(yaw), math.sin(pitch), math.cos(pitch) * math.cos(yaw)],
@@ -562,6 +577,61 @@ def small_scene(seed=0x5EED0000, width=512, height=512, n_tori=16):
     return s
 
 
+def default_scene(seed=0x5EED0005, width=320, height=180, n_tori=100, eye=(0.0, 3.5, -3.0), pitch=-0.75):
+    """The reference's default scene (src/scene.rs:95-156), seeded: a 20 x 20
+    ground plane (the 2 x 2 plane scaled by 10), a torus at (0, 1, 0), an
+    icosphere where the reference loads its glTF character, the double-sided
+    Portal plane at y = 1 (the 2 x 2 plane translated by (0, 1, 0), hit group
+    Portal: rays that hit it continue from the hit point moved up by 5), the
+    double-sided alpha-clipped fence at (2, 0, 2), and n_tori small tori at
+    T(U[-10,10], U[0.5,2.5], U[-10,10]) Ry(U[0,100)) S(U[0.1,0.3]), half
+    Textured and half Mirror. Camera: the reference's first-person camera
+    (yaw 180 degrees, 59 degree field of view, src/main.rs:568-593), raised and
+    pitched down so that the portal and the ground its rays reach are in view;
+    sun pitch 0.5, yaw 1.0, radius 0.05 (src/main.rs:577-584); 2 shadow rays
+    per hit as closest_hit_textured.glsl:195."""
+    rng = Rng(seed)
+    s = Scene("default")
+    s.seed = seed
+    _base_textures(s, seed & 0xFFFF)
+    checker = s.add_texture(checker_texture(256, 32))
+    plane = s.add_mesh(quad(2.0, 2.0, 1.0, "plane"))
+    t_mesh = s.add_mesh(torus(48, 12, 1.0, 0.35))
+    i_mesh = s.add_mesh(icosphere(3, 1.0))
+    fence_mesh = s.add_mesh(quad(2.0, 2.0, 2.0, "fence", opaque=False, vertical=True))
+    ground = s.add_model(plane, [s.material((0.55, 0.55, 0.5), 0.8)])
+    s.instances.append(InstanceData(affine((0.0, 0.0, 0.0), None, 10.0), ground, plane))
+    tm = s.add_model(t_mesh, [s.material((0.8, 0.3, 0.2), 0.4)])
+    s.instances.append(InstanceData(affine((0.0, 1.0, 0.0)), tm, t_mesh))
+    im = s.add_model(i_mesh, [s.material((0.7, 0.7, 0.75), 0.3, 1.0)])
+    s.instances.append(InstanceData(affine((-2.0, 0.5, -1.0), rotation_y(math.radians(150.0)), 0.5), im, i_mesh))
+    pm = s.add_model(plane, [s.material((1.0, 1.0, 1.0), 0.5)])
+    s.instances.append(InstanceData(affine((0.0, 1.0, 0.0)), pm, plane, abi.RTG_HIT_PORTAL,
+                                    abi.RTG_INSTANCE_TRIANGLE_FACING_CULL_DISABLE))
+    fm = s.add_model(fence_mesh, [(checker, s.material((1, 1, 1), 0.6)[1], -1)])
+    s.instances.append(InstanceData(affine((2.0, 0.0, 2.0)), fm, fence_mesh, abi.RTG_HIT_TEXTURED,
+                                    abi.RTG_INSTANCE_TRIANGLE_FACING_CULL_DISABLE))
+    pos = rng.uniform(3 * n_tori).reshape(n_tori, 3)
+    ang = rng.uniform(n_tori, 0.0, 100.0)
+    scl = rng.uniform(n_tori, 0.1, 0.3)
+    mirror = rng.uniform(n_tori) < 0.5
+    cols = rng.uniform(3 * n_tori).reshape(n_tori, 3)
+    for k in range(n_tori):
+        tr = (-10.0 + 20.0 * pos[k, 0], 0.5 + 2.0 * pos[k, 1], -10.0 + 20.0 * pos[k, 2])
+        m = s.add_model(t_mesh, [s.material(cols[k], 0.5)])
+        s.instances.append(InstanceData(affine(tr, rotation_y(ang[k]), scl[k]), m, t_mesh,
+                                        abi.RTG_HIT_MIRROR if mirror[k] else abi.RTG_HIT_TEXTURED))
+    s.width, s.height = width, height
+    s.view_inverse = fps_view_inverse(eye, pitch, math.pi)
+    s.proj_inverse = proj_inverse_reversed_z(math.radians(59.0), width / height)
+    s.sun_dir = sun_direction(0.5, 1.0)
+    s.sun_radius = 0.05
+    s.shadow_rays = 2
+    s.max_traces = 3
+    s.spp = 1
+    return s
+
+
 class AnimatedScene(Scene):
     """A scene whose object rotations are re-randomised every frame
     (the per-frame transform write + TLAS UPDATE of src/scene.rs:167-204)."""
@@ -692,6 +762,32 @@ def batch_workload(n_scenes=64, seed=0x5EED0003, tri_range=(1_000, 500_000), res
         w, h = resolutions[int(res[k])]
         out.append(batch_scene(seed + 1 + k, float(tris[k]), w, h, int(spp[k])))
     return out
+
+
+SERVE_RESOLUTIONS = ((256, 256), (1280, 720), (1920, 1080))
+
+
+def served_stream(n_scenes=32, n_requests=160, seed=0x5EED0004, resolutions=SERVE_RESOLUTIONS, spp_range=(1, 8),
+                  scene_size=(256, 256)):
+    """The served stream: n_scenes scenes of the small-scene generator stay
+    resident; each of n_requests requests names one of them, a resolution
+    from `resolutions`, samples per pixel uniform in spp_range (inclusive) and
+    new random rotations of every torus (a transform update and refit).
+    Returns (scenes, requests), a request being a dict with scene, width,
+    height, spp, frame and transforms (row-major 3x4 per instance)."""
+    scenes_ = [small_scene(seed=seed + 1 + k, width=scene_size[0], height=scene_size[1]) for k in range(n_scenes)]
+    rng = Rng(seed)
+    reqs = []
+    for i in range(n_requests):
+        sid = int(rng.integers(1, 0, n_scenes)[0])
+        w, h = resolutions[int(rng.integers(1, 0, len(resolutions))[0])]
+        spp = int(rng.integers(1, spp_range[0], spp_range[1] + 1)[0])
+        xf = [ins.transform.copy() for ins in scenes_[sid].instances]
+        rot = rng.uniform(3 * len(xf)).reshape(-1, 3)
+        for k in range(1, len(xf)):
+            xf[k] = affine(xf[k][:, 3], rotation_from_uniform(*rot[k]), 1.0)
+        reqs.append({"scene": sid, "width": w, "height": h, "spp": spp, "frame": i, "transforms": xf})
+    return scenes_, reqs
 
 
 def incoherent_rays(n_rays, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), seed=0x5EED0002):

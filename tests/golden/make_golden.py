@@ -7,6 +7,9 @@ regenerating a scene on another machine:
   large_mini.npz     reduced animated scene (tori, icospheres, soups, alpha
                      fences, mirrors), 96x54, 4 soft-shadow rays, frame 3
   small_spp.npz      small scene, 64x48, 3 samples, 2 shadow rays, frame 7
+  default_portal.npz the reference's default scene layout with its Portal
+                     plane (continuation rays), mirrors and alpha fence,
+                     160x90, 2 soft-shadow rays
   rays_small.npz     incoherent closest-hit and occlusion queries
 Run: python tests/golden/make_golden.py
 """
@@ -47,6 +50,7 @@ def main():
     s.shadow_rays = 2
     s.frame_index = 7
     frame_fixture("small_spp.npz", s)
+    frame_fixture("default_portal.npz", scenes.default_scene(width=160, height=90))
     # ray queries
     s = scenes.small_scene(width=16, height=16)
     hs = oh.HostScene(s)

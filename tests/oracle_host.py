@@ -54,6 +54,18 @@ def lib():
         L.orc_watertight.argtypes = [fp, fp, fp, fp, fp, C.c_float, C.c_float, C.POINTER(C.c_int), fp, fp, fp]
         L.orc_morton3.argtypes = [C.c_float] * 3 + [fp, fp]
         L.orc_morton3.restype = C.c_uint32
+        L.orc_camera_ray.argtypes = [C.POINTER(abi.Uniforms), u32, u32, u32, u32, C.c_float, C.c_float, fp, fp]
+        L.orc_primary_miss.argtypes = [fp, fp, C.c_float, fp]
+        L.orc_terminator_fix.argtypes = [fp, fp, fp, fp]
+        L.orc_blue_noise.argtypes = [C.POINTER(abi.Texture), fp, u32, u32, u32, u32, fp]
+        L.orc_shading_normal.argtypes = [fp, fp, fp, fp, fp, fp]
+        L.orc_textured_colour.argtypes = [fp, fp, fp, C.c_float, fp, C.c_float, C.c_float, fp]
+        L.orc_mirror.argtypes = [fp, fp, fp, fp, C.c_float, fp, fp]
+        L.orc_portal.argtypes = [fp, fp, C.c_float, fp, fp]
+        L.orc_heatmap_pixels.argtypes = [vp, vp, u32, vp]
+        for name in ("orc_camera_ray", "orc_primary_miss", "orc_terminator_fix", "orc_blue_noise",
+                     "orc_shading_normal", "orc_textured_colour", "orc_mirror", "orc_portal", "orc_heatmap_pixels"):
+            getattr(L, name).restype = None
         _lib = L
     return _lib
 

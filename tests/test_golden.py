@@ -10,7 +10,7 @@ import oracle_host as oh
 from rtg import abi, scenes
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-FRAMES = ["small_frame.npz", "large_mini.npz", "small_spp.npz"]
+FRAMES = ["small_frame.npz", "large_mini.npz", "small_spp.npz", "default_portal.npz"]
 
 
 def load(name):
