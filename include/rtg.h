@@ -52,7 +52,10 @@ typedef enum rtg_status {
     RTG_ERROR_LIMIT = -4,
     RTG_ERROR_NOT_FOUND = -5,
     RTG_ERROR_BUDGET = -6,
-    RTG_ERROR_STATE = -7
+    RTG_ERROR_STATE = -7,
+    /* a served request's trace ran but set RTG_STATUS_STACK_OVERFLOW or
+       RTG_STATUS_QUEUE_RANGE: its outputs are not valid (rtg_completion.status) */
+    RTG_ERROR_RESULT = -8
 } rtg_status;
 
 /* ------------------------------------------------------------------------ */
@@ -223,7 +226,7 @@ int rtg_tlas_destroy(rtg_ctx* ctx, rtg_tlas tlas);
 /* Statistics of a built structure (host copy of its header; synchronises). */
 typedef struct rtg_as_info {
     uint32_t kind;        /* 0 = BLAS, 1 = TLAS */
-    uint32_t node_count;  /* 64-byte nodes */
+    uint32_t node_count;  /* 128-byte 4-wide nodes */
     uint32_t prim_count;  /* triangles (BLAS) or instances (TLAS) */
     uint32_t geometry_count;
     uint32_t max_depth;   /* longest root-to-leaf path, in nodes */
@@ -358,7 +361,7 @@ typedef struct rtg_trace_counters {
     uint64_t bounce_rays;      /* mirror/portal continuation rays */
     uint64_t shadow_rays;      /* occlusion rays */
     /* closest-hit traversals */
-    uint64_t node_visits;      /* 64-byte node records fetched */
+    uint64_t node_visits;      /* 128-byte node records fetched */
     uint64_t triangle_tests;   /* triangle records tested */
     uint64_t instance_visits;  /* TLAS leaf -> BLAS transitions */
     uint64_t anyhit_calls;     /* alpha-clip evaluations */
@@ -381,6 +384,13 @@ typedef struct rtg_trace_counters {
     uint64_t side_triangle_tests;
     uint64_t side_instance_visits;
     uint64_t side_anyhit_calls;
+    /* traversal-stack pushes that went past the part of the stack in LDS (16
+       closest-hit entries, 32 occlusion entries per lane) to the lane's spill
+       area in device memory; the side_* pair is the bounce stream's share */
+    uint64_t spill_pushes;
+    uint64_t occ_spill_pushes;
+    uint64_t side_spill_pushes;
+    uint64_t side_occ_spill_pushes;
 } rtg_trace_counters;
 /* Enables traversal counters (slower kernels) for subsequent traces. */
 int rtg_set_counters(rtg_ctx* ctx, int enable);
@@ -493,7 +503,11 @@ int rtg_batch_destroy(rtg_batch* batch);
 typedef struct rtg_server rtg_server;
 
 typedef struct rtg_server_config {
-    uint64_t hbm_budget_bytes;  /* frame state + TLAS copies of all in-flight batches */
+    uint64_t hbm_budget_bytes;  /* device memory of all slots: each of the max_inflight slots holds the spill area
+                                   of the persistent traversal grid (cu_count * blocks_per_cu * 256 lanes * 640 B)
+                                   and the frame state and TLAS copies of its batches within its share
+                                   (budget / max_inflight); rtg_server_create refuses (RTG_ERROR_BUDGET) a budget
+                                   whose share cannot hold a spill area */
     uint32_t max_batch_jobs;    /* requests per batch; 1 = one request per launch */
     uint32_t max_inflight;      /* batches in flight at once (1..8) */
     uint64_t max_batch_samples; /* samples (pixels x spp) per batch, the first request excepted; 0 = no cap.
@@ -518,7 +532,8 @@ typedef struct rtg_request {
 typedef struct rtg_completion {
     uint64_t ticket;
     uint64_t tag;
-    int32_t status;    /* RTG_OK or the launch error */
+    int32_t status;    /* RTG_OK, RTG_ERROR_RESULT (the trace overflowed a traversal stack or dropped rays) or
+                          the launch error */
     uint32_t batch_size;
     double submit_s;   /* server clock (seconds since rtg_server_create) */
     double start_s;    /* its batch was launched */
@@ -531,9 +546,10 @@ typedef struct rtg_server_stats {
     uint64_t rejected;
     uint64_t max_batch;
     uint64_t max_inflight_seen;
-    uint64_t peak_bytes;
+    uint64_t peak_bytes; /* most device bytes the slots held at once (frame state, spill areas, TLAS copies) */
 } rtg_server_stats;
 
+/* The server traces with the context's launch tuning of this moment. */
 int rtg_server_create(rtg_ctx* ctx, const rtg_server_config* cfg, rtg_server** out);
 /* Registers a resident scene: its TLAS (the template copied per request) and
  * its bindings. Scenes stay resident until the server is destroyed. */

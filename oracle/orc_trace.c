@@ -267,6 +267,11 @@ static int traverse(qctx* q, v3 o, v3 d, float tmin, float tmax, int occlusion, 
     best->inst = best->prim = best->geom = RTG_NO_HIT;
     /* counters of this query kind: nodes, triangles, instances, any-hits */
     uint64_t* kc = occlusion ? &q->cnt.occ_node_visits : &q->cnt.node_visits;
+    /* pushes at stack positions the device keeps in its per-lane spill area
+       (beyond 16 closest-hit or 32 occlusion entries in LDS); the push
+       order, hence every position, is the device's */
+    uint64_t* ksp = occlusion ? &q->cnt.occ_spill_pushes : &q->cnt.spill_pushes;
+    const uint32_t lds_depth = occlusion ? 32u : 16u;
     uint32_t cur = 0;
     int have = 1;
     while (1) {
@@ -300,6 +305,7 @@ static int traverse(qctx* q, v3 o, v3 d, float tmin, float tmax, int occlusion, 
                 }
                 for (int k = nh - 1; k >= 1; --k) {
                     if (sp >= STACK_MAX) return -1;
+                    if (sp >= lds_depth) ++*ksp;
                     stack_ref[sp] = ref[k];
                     stack_tn[sp] = tn[k];
                     ++sp;
@@ -315,6 +321,7 @@ static int traverse(qctx* q, v3 o, v3 d, float tmin, float tmax, int occlusion, 
                 if (!bh) continue;
                 kc[2]++;
                 if (sp >= STACK_MAX) return -1;
+                if (sp >= lds_depth) ++*ksp;
                 stack_ref[sp] = SENTINEL;
                 stack_tn[sp] = -INFINITY;
                 ++sp;
@@ -409,6 +416,8 @@ static void add_counters(rtg_trace_counters* dst, const rtg_trace_counters* s) {
     dst->occ_triangle_tests += s->occ_triangle_tests;
     dst->occ_instance_visits += s->occ_instance_visits;
     dst->occ_anyhit_calls += s->occ_anyhit_calls;
+    dst->spill_pushes += s->spill_pushes;
+    dst->occ_spill_pushes += s->occ_spill_pushes;
 }
 
 /* ---------------------------------------------------------------- */

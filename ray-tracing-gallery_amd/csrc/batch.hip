@@ -191,12 +191,18 @@ struct rtg_batch {
     std::vector<uint32_t> shape;  // width, height, spp, shadow_rays, max_traces per job
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // completion of the last launch: a launch on another stream waits for it,
+    // since every launch uses the batch's one frame state
+    hipEvent_t done = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool launched = false;
 };
 
 static void batch_free(rtg_batch* b) {
     if (!b) return;
     hipSetDevice(b->ctx->device);
     hipDeviceSynchronize();
+    if (b->done) hipEventDestroy(b->done);
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
     delete_frame_arena(b->arena);
@@ -289,8 +295,15 @@ extern "C" int rtg_batch_launch(rtg_batch* b, void* stream) {
     RTG_REQUIRE(b, RTG_ERROR_INVALID_ARGUMENT, "rtg_batch_launch: null batch");
     RTG_HIP(hipSetDevice(b->ctx->device));
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    LaunchTimer t(b->ctx, RTG_KERNEL_BATCH_GRAPH, s);
-    RTG_HIP(hipGraphLaunch(b->exec, s));
+    if (!b->done) RTG_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+    if (b->launched && s != b->last_stream) RTG_HIP(hipStreamWaitEvent(s, b->done, 0));
+    {
+        LaunchTimer t(b->ctx, RTG_KERNEL_BATCH_GRAPH, s);
+        RTG_HIP(hipGraphLaunch(b->exec, s));
+    }
+    RTG_HIP(hipEventRecord(b->done, s));
+    b->last_stream = s;
+    b->launched = true;
     return RTG_OK;
 }
 

@@ -20,7 +20,7 @@ void set_error(const char* fmt, ...) {
 }
 void clear_error() { g_error[0] = 0; }
 
-int DeviceBuffer::ensure(size_t bytes) {
+int DeviceBuffer::ensure(size_t bytes, bool slack) {
     if (bytes <= size) return RTG_OK;
     if (ptr) {
         hipError_t e = hipDeviceSynchronize();
@@ -32,7 +32,7 @@ int DeviceBuffer::ensure(size_t bytes) {
         ptr = nullptr;
         size = 0;
     }
-    size_t want = bytes + bytes / 4;
+    size_t want = slack ? bytes + bytes / 4 : bytes;
     hipError_t e = hipMalloc(&ptr, want);
     if (e != hipSuccess) {
         ptr = nullptr;
@@ -183,6 +183,8 @@ extern "C" int rtg_get_counters(rtg_ctx* ctx, rtg_trace_counters* out) {
     out->triangle_tests += out->side_triangle_tests;
     out->instance_visits += out->side_instance_visits;
     out->anyhit_calls += out->side_anyhit_calls;
+    out->spill_pushes += out->side_spill_pushes;
+    out->occ_spill_pushes += out->side_occ_spill_pushes;
     return RTG_OK;
 }
 

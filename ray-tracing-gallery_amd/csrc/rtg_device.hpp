@@ -460,7 +460,7 @@ struct HitRec {
 };
 
 struct TravCounters {
-    uint32_t nodes, tris, insts, anyhits;
+    uint32_t nodes, tris, insts, anyhits, spills;
 };
 
 struct TlasView {
@@ -515,33 +515,79 @@ __device__ inline bool alpha_clip_keep(const SceneView& S, const rtg_instance* s
     return !(tx.w < 0.5f);
 }
 
-// Per-lane traversal stack: the first kLdsDepth entries live in LDS
-// (entry-major so that lanes never conflict), deeper entries in a per-thread
-// global spill area of kSpillDepth entries.
-constexpr int kLdsDepth = 16;
+// Per-lane traversal stack: the first entries live in LDS (entry-major so
+// that lanes never conflict), deeper ones in a per-lane global spill area of
+// kSpillLaneBytes. A closest-hit entry is 8 B, a node reference and its entry
+// distance (popped entries beyond the closest hit are skipped); an occlusion
+// entry is the 4-B reference alone, since an occlusion query never shortens
+// its interval (the first accepted hit ends it), so twice as many fit in the
+// same 32 KB of LDS per 256-lane workgroup. Both hold kStackMax entries.
+constexpr int kLdsDepth = 16;     // closest-hit entries in LDS
+constexpr int kLdsDepthOcc = 32;  // occlusion entries in LDS
 constexpr int kSpillDepth = 80;
 constexpr int kStackMax = kLdsDepth + kSpillDepth;
 // bytes of one lane's spill area
 constexpr size_t kSpillLaneBytes = (size_t)kSpillDepth * 8;
+// LDS bytes of a 256-lane workgroup's stacks (either kind)
+constexpr size_t kStackLdsBytes = (size_t)kLdsDepth * 8 * 256;
+static_assert(kLdsDepthOcc * 4 * 256 == kStackLdsBytes, "both stack kinds take the same LDS");
+static_assert((kStackMax - kLdsDepthOcc) * 4 <= (int)kSpillLaneBytes, "occlusion spill fits the lane's area");
 
-struct Stack {
+template <bool OCC>
+struct Stack;
+
+template <>
+struct Stack<false> {
+    static constexpr int kLds = kLdsDepth;
     uint2* lds;      // lds[k * stride + lane]
-    uint2* spill;    // spill[k - kLdsDepth], per thread
+    uint2* spill;    // spill[k - kLds], per lane
     int stride;
     int sp;
+    __device__ inline void init(void* lds_words, uint8_t* spill_area, uint32_t gtid, int lane, int lanes) {
+        lds = static_cast<uint2*>(lds_words) + lane;
+        stride = lanes;
+        spill = reinterpret_cast<uint2*>(spill_area + (size_t)gtid * kSpillLaneBytes);
+        sp = 0;
+    }
     __device__ inline void push(uint32_t ref, float tn) {
         uint2 e = make_uint2(ref, f2u(tn));
-        if (sp < kLdsDepth) {
+        if (sp < kLds) {
             lds[sp * stride] = e;
         } else {
             rtg_u2v v = {e.x, e.y};
-            *(RTG_GLOBAL rtg_u2v*)(spill + (sp - kLdsDepth)) = v;
+            *(RTG_GLOBAL rtg_u2v*)(spill + (sp - kLds)) = v;
         }
         ++sp;
     }
     __device__ inline uint2 pop() {
         --sp;
-        return sp < kLdsDepth ? lds[sp * stride] : gld_u2(spill + (sp - kLdsDepth));
+        return sp < kLds ? lds[sp * stride] : gld_u2(spill + (sp - kLds));
+    }
+};
+
+template <>
+struct Stack<true> {
+    static constexpr int kLds = kLdsDepthOcc;
+    uint32_t* lds;
+    uint32_t* spill;
+    int stride;
+    int sp;
+    __device__ inline void init(void* lds_words, uint8_t* spill_area, uint32_t gtid, int lane, int lanes) {
+        lds = static_cast<uint32_t*>(lds_words) + lane;
+        stride = lanes;
+        spill = reinterpret_cast<uint32_t*>(spill_area + (size_t)gtid * kSpillLaneBytes);
+        sp = 0;
+    }
+    __device__ inline void push(uint32_t ref, float) {
+        if (sp < kLds) lds[sp * stride] = ref;
+        else *(RTG_GLOBAL uint32_t*)(spill + (sp - kLds)) = ref;
+        ++sp;
+    }
+    // (reference, -inf): an occlusion entry is never skipped
+    __device__ inline uint2 pop() {
+        --sp;
+        uint32_t r = sp < kLds ? lds[sp * stride] : gld_u32(spill + (sp - kLds));
+        return make_uint2(r, 0xff800000u);
     }
 };
 
@@ -570,8 +616,10 @@ struct Traverser {
     bool in_blas;
     HitRec best;  // best.inst stays RTG_NO_HIT until a hit is accepted
 
+    using StackT = Stack<OCCLUSION>;
+
     __device__ inline void start(const TlasView& T, V3 o, V3 d, float tmin_, float tmax_, uint32_t flags,
-                                 uint32_t mask, uint32_t sbt, Stack& st) {
+                                 uint32_t mask, uint32_t sbt, StackT& st) {
         ray_setup_box(world, o, d);
         in_blas = false;
         nodes = T.nodes;
@@ -592,7 +640,7 @@ struct Traverser {
     __device__ inline int done() const { return best.inst != RTG_NO_HIT ? kRayHit : kRayMiss; }
 
     // Next entry to visit; false once the stack is empty.
-    __device__ inline bool pop(const TlasView& T, Stack& st) {
+    __device__ inline bool pop(const TlasView& T, StackT& st) {
         while (st.sp > 0) {
             uint2 e = st.pop();
             if (e.x == kSentinel) {
@@ -600,14 +648,20 @@ struct Traverser {
                 nodes = T.nodes;
                 continue;
             }
-            if (u2f(e.y) > tcur) continue;
+            if (!OCCLUSION && u2f(e.y) > tcur) continue;
             cur = e.x;
             return true;
         }
         return false;
     }
 
-    __device__ inline int step(const TlasView& T, const SceneView& S, Stack& st, TravCounters& cnt) {
+    // pushes from stack position sp0 on that went past the LDS part
+    __device__ static inline uint32_t spilled(int sp0, int sp1) {
+        const int from = sp0 > StackT::kLds ? sp0 : StackT::kLds;
+        return sp1 > from ? (uint32_t)(sp1 - from) : 0u;
+    }
+
+    __device__ inline int step(const TlasView& T, const SceneView& S, StackT& st, TravCounters& cnt) {
         if (!(cur & kLeafBit)) {
             const RayCtx& cr = in_blas ? obj : world;
             const float4* np = reinterpret_cast<const float4*>(nodes + cur);
@@ -627,6 +681,7 @@ struct Traverser {
                 return kRayActive;
             }
             if (st.sp + nh - 1 > kStackMax) return kRayOverflow;
+            if (COUNT) cnt.spills += spilled(st.sp, st.sp + nh - 1);
             if (OCCLUSION) {
                 // slot order: the lowest overlapped slot is visited next, the
                 // others are pushed highest first
@@ -665,6 +720,7 @@ struct Traverser {
             }
             if (COUNT) cnt.insts++;
             if (st.sp >= kStackMax) return kRayOverflow;
+            if (COUNT) cnt.spills += spilled(st.sp, st.sp + 1);
             st.push(kSentinel, -__builtin_inff());
             float w2o[12];
             float4 m0 = gld_f4(rp), m1 = gld_f4(rp + 1), m2 = gld_f4(rp + 2);
@@ -720,8 +776,8 @@ struct Traverser {
 // if the stack overflowed.
 template <bool OCCLUSION, bool COUNT>
 __device__ inline int traverse(const TlasView& T, const SceneView& S, V3 o, V3 d, float tmin, float tmax,
-                               uint32_t ray_flags, uint32_t cull_mask, uint32_t sbt_offset, HitRec& best, Stack& st,
-                               TravCounters& cnt) {
+                               uint32_t ray_flags, uint32_t cull_mask, uint32_t sbt_offset, HitRec& best,
+                               Stack<OCCLUSION>& st, TravCounters& cnt) {
     Traverser<OCCLUSION, COUNT> tr;
     tr.start(T, o, d, tmin, tmax, ray_flags, cull_mask, sbt_offset, st);
     int s;

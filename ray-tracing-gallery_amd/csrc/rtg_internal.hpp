@@ -44,7 +44,8 @@ struct AsRecord {
 struct DeviceBuffer {
     void* ptr = nullptr;
     size_t size = 0;
-    int ensure(size_t bytes);
+    // grows to at least `bytes` (with a quarter more headroom when `slack`)
+    int ensure(size_t bytes, bool slack = true);
     void release();
 };
 
@@ -151,7 +152,17 @@ struct LaunchTimer {
 };
 int check_status(rtg_ctx* ctx, hipStream_t stream, uint32_t mask, const char* what);
 void free_frame_arena(rtg_ctx* ctx);
-FrameArena* new_frame_arena();
+// exact: grow to exactly the sizes asked (a server slot's budgeted arena)
+FrameArena* new_frame_arena(bool exact = false);
+// Device bytes an arena's frame buffer has after being sized for a trace of
+// `paths` paths, `shadow` shadow rays and `njobs` jobs (it only grows).
+size_t arena_bytes_after(const FrameArena* a, uint64_t paths, uint64_t shadow, uint32_t njobs);
+// Device bytes an arena holds (frame buffer, spill area, feed counter).
+size_t arena_held_bytes(const FrameArena* a);
+// Frees an idle arena's frame buffer (its next trace allocates it anew).
+void arena_reset(FrameArena* a);
+// The device word with the status bits of the arena's last trace.
+const uint32_t* arena_status_word(const FrameArena* a);
 void delete_frame_arena(FrameArena* a);
 size_t frame_bytes(uint64_t paths, uint64_t shadow, uint32_t njobs);
 // The trace step of a batch of jobs on `stream` with the frame state in `a`

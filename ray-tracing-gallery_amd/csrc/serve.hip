@@ -46,6 +46,7 @@ struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     FrameArena* arena = nullptr;
+    uint32_t* h_status = nullptr;  // pinned copy of the status bits of the slot's last batch
     DeviceBuffer tlas_copies;
     bool busy = false;
     uint64_t bytes = 0;
@@ -60,7 +61,9 @@ using namespace rtg;
 struct rtg_server {
     rtg_ctx* ctx = nullptr;
     rtg_server_config cfg{};
-    uint64_t slot_capacity = 0;
+    Tuning tune;                 // the context's launch setting when the server was created
+    uint64_t slot_fixed = 0;     // device bytes every slot holds: its traversal spill area and feed counter
+    uint64_t slot_capacity = 0;  // device bytes a slot's batch may take beyond slot_fixed
     std::vector<ServerScene> scenes;
     std::vector<Slot> slots;
     std::deque<PendingRequest> pending;
@@ -78,15 +81,29 @@ struct rtg_server {
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
     void run();
     bool launch(Slot& s);
+    // device bytes held by all slots (frame buffers, spill areas, TLAS copies)
+    uint64_t held_bytes() const {
+        uint64_t b = 0;
+        for (const Slot& sl : slots) b += arena_held_bytes(sl.arena) + sl.tlas_copies.size;
+        return b;
+    }
 };
 
 namespace {
 
+// Device bytes of a request traced alone: its frame state and TLAS copy
+// (the slot's spill area is charged apart, once per slot).
 uint64_t request_bytes(const rtg_request& r, const ServerScene& sc) {
     uint64_t paths = (uint64_t)r.width * r.height * r.opts.spp;
     uint64_t bytes = frame_bytes(paths, paths * r.opts.shadow_rays, 1);
     if (r.instances) bytes += (sc.tlas_bytes + 255) & ~uint64_t(255);
     return bytes;
+}
+
+// Status of a completed request from its batch's device status bits.
+int32_t completion_status(hipError_t q, uint32_t bits) {
+    if (q != hipSuccess) return RTG_ERROR_DEVICE;
+    return (bits & (kStatusStackOverflow | kStatusQueueRange)) ? RTG_ERROR_RESULT : RTG_OK;
 }
 
 }  // namespace
@@ -95,9 +112,24 @@ uint64_t request_bytes(const rtg_request& r, const ServerScene& sc) {
 // launch error, which completes the batch with that status.
 bool rtg_server::launch(Slot& s) {
     size_t copy_bytes = 0;
-    for (auto& p : s.batch)
+    uint64_t paths = 0, shadow = 0;
+    for (auto& p : s.batch) {
         if (p.req.instances) copy_bytes += (p.scene.tlas_bytes + 255) & ~uint64_t(255);
-    if (copy_bytes && s.tlas_copies.ensure(copy_bytes) != RTG_OK) return false;
+        uint64_t rp = (uint64_t)p.req.width * p.req.height * p.req.opts.spp;
+        paths += rp;
+        shadow += rp * p.req.opts.shadow_rays;
+    }
+    // Buffers only grow, each to the largest batch in each dimension; when
+    // that would take the slot past its share of the budget, the idle slot's
+    // buffers are freed and allocated anew for this batch alone (which was
+    // admitted because it fits).
+    if (arena_bytes_after(s.arena, paths, shadow, (uint32_t)s.batch.size()) +
+            std::max<uint64_t>(copy_bytes, s.tlas_copies.size) >
+        slot_capacity) {
+        arena_reset(s.arena);
+        s.tlas_copies.release();
+    }
+    if (copy_bytes && s.tlas_copies.ensure(copy_bytes, false) != RTG_OK) return false;
     std::vector<rtg_image_job> jobs(s.batch.size());
     size_t off = 0;
     for (size_t k = 0; k < s.batch.size(); ++k) {
@@ -125,7 +157,11 @@ bool rtg_server::launch(Slot& s) {
         j.out_rgb = r.out_rgb;
         j.out_primary_hits = r.out_primary_hits;
     }
-    if (trace_jobs(ctx, jobs.data(), (uint32_t)jobs.size(), s.stream, s.arena, false) != RTG_OK) return false;
+    if (trace_jobs(ctx, jobs.data(), (uint32_t)jobs.size(), s.stream, s.arena, false, &tune) != RTG_OK)
+        return false;
+    if (hipMemcpyAsync(s.h_status, arena_status_word(s.arena), sizeof(uint32_t), hipMemcpyDeviceToHost, s.stream) !=
+        hipSuccess)
+        return false;
     return hipEventRecord(s.done, s.stream) == hipSuccess;
 }
 
@@ -145,7 +181,7 @@ void rtg_server::run() {
                 rtg_completion c{};
                 c.ticket = p.ticket;
                 c.tag = p.req.tag;
-                c.status = q == hipSuccess ? RTG_OK : RTG_ERROR_DEVICE;
+                c.status = completion_status(q, *s.h_status);
                 c.batch_size = (uint32_t)s.batch.size();
                 c.submit_s = p.submit_s;
                 c.start_s = s.start_s;
@@ -187,7 +223,6 @@ void rtg_server::run() {
             inflight_bytes += bytes;
             stats.batches++;
             stats.max_batch = std::max<uint64_t>(stats.max_batch, s.batch.size());
-            stats.peak_bytes = std::max<uint64_t>(stats.peak_bytes, inflight_bytes);
             uint64_t busy = 0;
             for (Slot& o : slots) busy += o.busy;
             stats.max_inflight_seen = std::max<uint64_t>(stats.max_inflight_seen, busy);
@@ -196,6 +231,7 @@ void rtg_server::run() {
             lk.unlock();
             bool ok = launch(s);
             lk.lock();
+            stats.peak_bytes = std::max<uint64_t>(stats.peak_bytes, held_bytes());
             if (!ok) {
                 // complete the batch with an error status
                 hipStreamSynchronize(s.stream);
@@ -243,19 +279,34 @@ extern "C" int rtg_server_create(rtg_ctx* ctx, const rtg_server_config* cfg, rtg
     RTG_REQUIRE(cfg->hbm_budget_bytes >= (64ull << 20), RTG_ERROR_INVALID_ARGUMENT,
                 "rtg_server_create: budget below 64 MiB");
     RTG_HIP(hipSetDevice(ctx->device));
+    // Every slot holds the spill area of the persistent traversal grid and a
+    // feed counter; the rest of its share of the budget is for the frame
+    // state and TLAS copies of its batches.
+    const uint64_t share = cfg->hbm_budget_bytes / cfg->max_inflight;
+    const uint64_t fixed = spill_bytes(ctx, ctx->tune) + 256;
+    const uint64_t smallest = frame_bytes(1, 1, 1);
+    RTG_REQUIRE(share >= fixed + smallest, RTG_ERROR_BUDGET,
+                "rtg_server_create: a budget of %llu bytes over %u slots leaves %llu per slot, below the %llu bytes "
+                "of a slot's traversal spill area and smallest frame state",
+                (unsigned long long)cfg->hbm_budget_bytes, cfg->max_inflight, (unsigned long long)share,
+                (unsigned long long)(fixed + smallest));
     rtg_server* s = new rtg_server();
     s->ctx = ctx;
     s->cfg = *cfg;
-    s->slot_capacity = cfg->hbm_budget_bytes / cfg->max_inflight;
+    s->tune = ctx->tune;
+    s->slot_fixed = fixed;
+    s->slot_capacity = share - fixed;
     s->slots.resize(cfg->max_inflight);
     for (Slot& sl : s->slots) {
+        sl.arena = new_frame_arena(true);
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
-            set_error("rtg_server_create: stream or event creation failed");
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void**>(&sl.h_status), sizeof(uint32_t)) != hipSuccess) {
+            set_error("rtg_server_create: stream, event or pinned status word creation failed");
             rtg_server_destroy(s);
             return RTG_ERROR_DEVICE;
         }
-        sl.arena = new_frame_arena();
+        *sl.h_status = 0;
     }
     s->t0 = std::chrono::steady_clock::now();
     s->worker = std::thread([s] { s->run(); });
@@ -383,6 +434,7 @@ extern "C" int rtg_server_destroy(rtg_server* srv) {
         if (s.stream) hipStreamSynchronize(s.stream);
         delete_frame_arena(s.arena);
         s.tlas_copies.release();
+        if (s.h_status) hipHostFree(s.h_status);
         if (s.done) hipEventDestroy(s.done);
         if (s.stream) hipStreamDestroy(s.stream);
     }

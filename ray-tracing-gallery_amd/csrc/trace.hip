@@ -11,6 +11,7 @@
 // with one atomic per wavefront (ballot/prefix-sum compaction). A path's
 // result depends only on its own rays, so queue order never changes results.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -26,6 +27,7 @@ constexpr int kFeedLevel = 8;    // counts[kFeedLevel + level]: feed counter of 
 constexpr int kFeedShadow = 12;  // counts[kFeedShadow]: feed counter of the shadow queue
 constexpr int kShadowFirst = 5;      // counts[5]: shadow rays of trace level 0
 constexpr int kFeedShadowRest = 13;  // counts[13]: feed counter of the shadow rays after them
+constexpr int kStatusSlot = 14;      // counts[14]: status bits of the trace (merged into the context's)
 constexpr int kPartAll = 0, kPartFirst = 1, kPartRest = 2;  // parts of the shadow queue
 
 // Order of an image's pixels along its paths: bands of 8 rows, each band
@@ -90,8 +92,8 @@ struct Frame {
     uint64_t paths;
     float4* path_rgb;    // final colour of paths that did not end on a textured hit
     float4* shade_a;     // NoL, combined.xyz of a textured hit
-    float4* shade_b;     // ambient.xyz, 1 if textured
-    uint32_t* vis;       // visible shadow rays
+    float4* shade_b;     // ambient.xyz; w: bits of 1 + the path's first shadow ray if textured, else 0
+    uint8_t* sflag;      // per shadow-queue entry: 1 if the shadow ray saw the sun
     uint32_t* cycles;    // heatmap cycle counts
     Queue q[2];
     uint4* hit_a;        // t bits, instance, primitive, geometry
@@ -159,13 +161,20 @@ constexpr int kCountersOcclusion = 7;
 constexpr int kCounterSideShadow = 11;
 constexpr int kCountersOcclusionSide = 12;
 constexpr int kCountersClosestSide = 16;
-constexpr int kCounterSlots = 20;
+// stack pushes past the LDS part: slots 20 (closest), 21 (occlusion) and,
+// for the bounce stream, 22 (closest) and 23 (occlusion)
+constexpr int kCounterSlots = 24;
+
+__device__ inline int spill_slot(int base) {
+    return base == kCountersClosest ? 20 : base == kCountersOcclusion ? 21 : base == kCountersClosestSide ? 22 : 23;
+}
 
 __device__ inline void flush_counters(uint64_t* counters, const TravCounters& c, int base) {
     wave_add_counter(&counters[base + 0], c.nodes);
     wave_add_counter(&counters[base + 1], c.tris);
     wave_add_counter(&counters[base + 2], c.insts);
     wave_add_counter(&counters[base + 3], c.anyhits);
+    wave_add_counter(&counters[spill_slot(base)], c.spills);
 }
 
 // Lanes of a persistent traversal kernel keep tracing rays until the queue is
@@ -188,11 +197,9 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_rays(const uint8_t* tla
     __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
     const TlasView T = tlas_view(tlas);
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
-    Stack st;
-    st.lds = s_stack + threadIdx.x;
-    st.stride = kTraceThreads;
-    st.spill = spill + (size_t)gtid * kSpillDepth;
-    TravCounters cnt = {0, 0, 0, 0};
+    Stack<OCC> st;
+    st.init(s_stack, reinterpret_cast<uint8_t*>(spill), gtid, threadIdx.x, kTraceThreads);
+    TravCounters cnt = {0, 0, 0, 0, 0};
     Traverser<OCC, COUNT> tr;
     WaveFeeder feeder;
     feeder.init(feed_chunk);
@@ -243,6 +250,11 @@ __global__ void k_frame_begin(uint32_t* counts, uint64_t* counters, bool reset_c
     if (reset_counters && threadIdx.x < kCounterSlots) counters[threadIdx.x] = 0;
 }
 
+// ORs a trace's status bits into the context's sticky ones.
+__global__ void k_status_merge(const uint32_t* trace_status, uint32_t* ctx_status) {
+    if (threadIdx.x == 0 && *trace_status) atomicOr(ctx_status, *trace_status);
+}
+
 __global__ void k_shadow_snapshot(uint32_t* counts) {
     if (threadIdx.x == 0) counts[kShadowFirst] = counts[4];
 }
@@ -288,7 +300,6 @@ __global__ __launch_bounds__(256) void k_raygen(Frame F, bool count) {
         F.q[0].path[p] = (uint32_t)p;
         F.path_rgb[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         F.shade_b[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        F.vis[p] = 0;
         F.cycles[p] = 0;
     }
     if (count && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long*)&F.counters[0], F.paths);
@@ -306,11 +317,9 @@ __device__ __forceinline__ void trace_closest(Frame F, int level, const uint32_t
     const uint32_t n = min(count_ptr ? *count_ptr : fixed_count, F.path_cap);
     const Queue Q = F.q[level & 1];
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
-    Stack st;
-    st.lds = s_stack + threadIdx.x;
-    st.stride = kTraceThreads;
-    st.spill = F.spill + (size_t)gtid * kSpillDepth;
-    TravCounters cnt = {0, 0, 0, 0};
+    Stack<false> st;
+    st.init(s_stack, reinterpret_cast<uint8_t*>(F.spill), gtid, threadIdx.x, kTraceThreads);
+    TravCounters cnt = {0, 0, 0, 0, 0};
     Traverser<false, COUNT> tr;
     WaveFeeder feeder;
     feeder.init(F.feed_chunk);
@@ -387,11 +396,9 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest_rounds(Frame F,
     const uint32_t n = min(count_ptr ? *count_ptr : fixed_count, F.path_cap);
     const Queue Q = F.q[level & 1];
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
-    Stack st;
-    st.lds = s_stack + threadIdx.x;
-    st.stride = kTraceThreads;
-    st.spill = F.spill + (size_t)gtid * kSpillDepth;
-    TravCounters cnt = {0, 0, 0, 0};
+    Stack<false> st;
+    st.init(s_stack, reinterpret_cast<uint8_t*>(F.spill), gtid, threadIdx.x, kTraceThreads);
+    TravCounters cnt = {0, 0, 0, 0, 0};
     const uint32_t stride = gridDim.x * kTraceThreads;
     const uint32_t rounds = (n + stride - 1) / stride;
     for (uint32_t r = 0; r < rounds; ++r) {
@@ -509,7 +516,7 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
         bool active = i < n;
         uint32_t want_shadow = 0;
         bool want_next = false;
-        V3 sh_origin = v3(0, 0, 0), next_o = v3(0, 0, 0), next_d = v3(0, 0, 0);
+        V3 sh_origin = v3(0, 0, 0), next_o = v3(0, 0, 0), next_d = v3(0, 0, 0), amb = v3(0, 0, 0);
         V3 sun = v3(0, 0, 0);
         float sun_radius = 0.0f;
         uint32_t p = 0, px = 0, py = 0, frame = 0, bn_index = 0;
@@ -615,7 +622,7 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
                     V3 comb;
                     brdf_parts(normal, v3(-d.x, -d.y, -d.z), sun, roughness, base, metallic, 0.5f, NoL, comb);
                     F.shade_a[p] = make_float4(NoL, comb.x, comb.y, comb.z);
-                    F.shade_b[p] = make_float4(0.1f * base.x, 0.1f * base.y, 0.1f * base.z, 1.0f);
+                    amb = v3(0.1f * base.x, 0.1f * base.y, 0.1f * base.z);
                 } else if (group == RTG_HIT_MIRROR) {
                     // closest_hit_mirror (closest_hit_mirror.glsl:11-29)
                     const HitGeom gi = load_hit_geom(S, custom, geom, prim);
@@ -647,6 +654,8 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
             queue_range_error(F);
             want_shadow = 0;
         }
+        // a textured hit's ambient term and where its shadow rays' flags lie
+        if (want_shadow) F.shade_b[p] = make_float4(amb.x, amb.y, amb.z, u2f(sbase + 1u));
         if (want_next) {
             NQ.o[nbase] = make_float4(next_o.x, next_o.y, next_o.z, 0.01f);
             NQ.d[nbase] = make_float4(next_d.x, next_d.y, next_d.z, 10000.0f);
@@ -721,16 +730,14 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5
     shadow_part(F, part, lo, hi, feed);
     const uint32_t n = hi - lo;
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
-    Stack st;
-    st.lds = s_stack + threadIdx.x;
-    st.stride = kTraceThreads;
-    st.spill = F.spill + (size_t)gtid * kSpillDepth;
-    TravCounters cnt = {0, 0, 0, 0};
+    Stack<true> st;
+    st.init(s_stack, reinterpret_cast<uint8_t*>(F.spill), gtid, threadIdx.x, kTraceThreads);
+    TravCounters cnt = {0, 0, 0, 0, 0};
     Traverser<true, COUNT> tr;
     WaveFeeder feeder;
     feeder.init(F.feed_chunk);
     bool active = false;
-    uint32_t path = 0;
+    uint32_t slot = 0;
     const JobDev* J = nullptr;
     TlasView T;
     uint32_t c0 = 0;
@@ -740,7 +747,8 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5
             uint32_t id = feeder.feed(idle, !active, feed, n);
             if (id != ~0u) {
                 id += lo;
-                path = F.sq.path[id];
+                slot = id;
+                uint32_t path = F.sq.path[id];
                 if (path >= F.paths) {
                     queue_range_error(F);
                     path = 0;
@@ -759,9 +767,13 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5
             int res = tr.step(T, J->scene, st, cnt);
             if (res != kRayActive) {
                 if (res == kRayOverflow) atomicOr(status, (uint32_t)kStatusStackOverflow);
-                if (J->flags & RTG_TRACE_HEATMAP) atomicAdd(&F.cycles[path], (uint32_t)__builtin_readcyclecounter() - c0);
-                // shadow_ray_miss (lib.rs:33-36): a miss counts as visible
-                if (res == kRayMiss) atomicAdd(&F.vis[path], 1u);
+                if (J->flags & RTG_TRACE_HEATMAP) {
+                    const uint32_t path = F.sq.path[slot];
+                    atomicAdd(&F.cycles[path < F.paths ? path : 0u], (uint32_t)__builtin_readcyclecounter() - c0);
+                }
+                // shadow_ray_miss (lib.rs:33-36): a miss counts as visible;
+                // one byte per shadow ray, plain stores (no atomics)
+                F.sflag[slot] = res == kRayMiss ? 1 : 0;
                 active = false;
             }
         }
@@ -777,11 +789,9 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion_rounds(Frame 
     shadow_part(F, part, lo, hi, feed);
     const uint32_t n = hi - lo;
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
-    Stack st;
-    st.lds = s_stack + threadIdx.x;
-    st.stride = kTraceThreads;
-    st.spill = F.spill + (size_t)gtid * kSpillDepth;
-    TravCounters cnt = {0, 0, 0, 0};
+    Stack<true> st;
+    st.init(s_stack, reinterpret_cast<uint8_t*>(F.spill), gtid, threadIdx.x, kTraceThreads);
+    TravCounters cnt = {0, 0, 0, 0, 0};
     const uint32_t stride = gridDim.x * kTraceThreads;
     const uint32_t rounds = (n + stride - 1) / stride;
     for (uint32_t r = 0; r < rounds; ++r) {
@@ -800,7 +810,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion_rounds(Frame 
                                             RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RTG_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER,
                                             0xffu, 1u, h, st, cnt);
             if (res < 0) atomicOr(status, (uint32_t)kStatusStackOverflow);
-            if (res == 0) atomicAdd(&F.vis[p], 1u);
+            F.sflag[i] = res == 0 ? 1 : 0;
         }
     }
     if (COUNT) flush_counters(F.counters, cnt, SIDE ? kCountersOcclusionSide : kCountersOcclusion);
@@ -851,9 +861,12 @@ __global__ __launch_bounds__(256) void k_resolve(Frame F, uint64_t total_pixels)
             uint64_t p = J.sample_base + (uint64_t)t * J.spp + s;
             float4 b = F.shade_b[p];
             V3 c;
-            if (b.w != 0.0f) {
+            const uint32_t first = f2u(b.w);
+            if (first != 0) {
                 float4 a = F.shade_a[p];
-                float sf = (float)F.vis[p] / (float)J.shadow_rays;
+                uint32_t vis = 0;
+                for (uint32_t k = 0; k < J.shadow_rays; ++k) vis += F.sflag[first - 1u + k];
+                float sf = (float)vis / (float)J.shadow_rays;
                 float in_nol = sf * a.x;
                 c = v3(a.y * in_nol + b.x, a.z * in_nol + b.y, a.w * in_nol + b.z);
             } else {
@@ -906,9 +919,19 @@ struct FrameArena {
     Frame F;
     JobDev* d_jobs = nullptr;
     uint32_t* d_counts = nullptr;
+    // grow to exactly the size asked (no headroom): a server slot's arena,
+    // whose device bytes count against an HBM budget
+    bool exact = false;
+    // host calls using the arena now: a second one at the same time (two
+    // threads tracing on one stream of one context) is refused
+    std::atomic<int> users{0};
 };
 
-FrameArena* new_frame_arena() { return new FrameArena(); }
+FrameArena* new_frame_arena(bool exact) {
+    FrameArena* a = new FrameArena();
+    a->exact = exact;
+    return a;
+}
 
 void delete_frame_arena(FrameArena* a) {
     if (!a) return;
@@ -938,20 +961,28 @@ void free_frame_arena(rtg_ctx* ctx) {
     ctx->frames.clear();
 }
 
-// Device bytes of the frame state of `paths` paths and `shadow` shadow rays.
+// Device bytes of the frame state with room for `paths` paths, `shadow`
+// shadow rays and `njobs` jobs: the buffer ensure_frame carves.
 size_t frame_bytes(uint64_t paths, uint64_t shadow, uint32_t njobs) {
     size_t off = 0;
-    carve_size<float4>(off, paths * 3);
-    carve_size<uint32_t>(off, paths * 2);
-    carve_size<float4>(off, paths * 4);
-    carve_size<uint32_t>(off, paths * 2);
+    carve_size<float4>(off, paths);    // path_rgb
+    carve_size<float4>(off, paths);    // shade_a
+    carve_size<float4>(off, paths);    // shade_b
+    carve_size<uint8_t>(off, shadow);  // sflag
+    carve_size<uint32_t>(off, paths);  // cycles
+    for (int q = 0; q < 2; ++q) {
+        carve_size<float4>(off, paths);
+        carve_size<float4>(off, paths);
+        carve_size<uint32_t>(off, paths);
+    }
     carve_size<uint4>(off, paths);
     carve_size<float2>(off, paths);
-    carve_size<float4>(off, shadow * 2);
+    carve_size<float4>(off, shadow);
+    carve_size<float4>(off, shadow);
     carve_size<uint32_t>(off, shadow);
     carve_size<uint32_t>(off, kCountSlots);
     carve_size<JobDev>(off, njobs);
-    return off + 16 * 256 + 4096;
+    return off + 4096;
 }
 
 static int trace_grid(rtg_ctx* ctx, const Tuning& tune) {
@@ -966,9 +997,27 @@ size_t spill_bytes(rtg_ctx* ctx, const Tuning& tune) {
 }
 
 static int ensure_spill(rtg_ctx* ctx, FrameArena* a, const Tuning& tune) {
-    if (a->spill.ensure(spill_bytes(ctx, tune)) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
-    return a->feed.ensure(256);
+    if (a->spill.ensure(spill_bytes(ctx, tune), !a->exact) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
+    return a->feed.ensure(256, !a->exact);
 }
+
+size_t arena_bytes_after(const FrameArena* a, uint64_t paths, uint64_t shadow, uint32_t njobs) {
+    if (a->buf.ptr && paths <= a->paths_cap && shadow <= a->shadow_cap && njobs <= a->jobs_cap) return a->buf.size;
+    size_t want = frame_bytes(std::max<uint64_t>(paths, a->paths_cap), std::max<uint64_t>(shadow, a->shadow_cap),
+                              std::max<uint32_t>(njobs, a->jobs_cap));
+    if (!a->exact) want += want / 4;
+    return std::max(want, a->buf.size);
+}
+
+size_t arena_held_bytes(const FrameArena* a) { return a->buf.size + a->spill.size + a->feed.size; }
+
+void arena_reset(FrameArena* a) {
+    a->buf.release();
+    a->paths_cap = a->shadow_cap = 0;
+    a->jobs_cap = 0;
+}
+
+const uint32_t* arena_status_word(const FrameArena* a) { return a->d_counts + kStatusSlot; }
 
 // Frame state of the context's calls on stream `s`: calls on different
 // streams (frames in flight) never share queues, counters or spill areas.
@@ -985,31 +1034,13 @@ static int ensure_frame(rtg_ctx* ctx, FrameArena* a, uint64_t paths, uint64_t sh
         uint64_t P = std::max<uint64_t>(paths, a->paths_cap);
         uint64_t S = std::max<uint64_t>(shadow, a->shadow_cap);
         uint32_t NJ = std::max<uint32_t>(njobs, a->jobs_cap);
-        size_t off = 0;
-        carve_size<float4>(off, P);        // path_rgb
-        carve_size<float4>(off, P);        // shade_a
-        carve_size<float4>(off, P);        // shade_b
-        carve_size<uint32_t>(off, P);      // vis
-        carve_size<uint32_t>(off, P);      // cycles
-        for (int q = 0; q < 2; ++q) {
-            carve_size<float4>(off, P);
-            carve_size<float4>(off, P);
-            carve_size<uint32_t>(off, P);
-        }
-        carve_size<uint4>(off, P);
-        carve_size<float2>(off, P);
-        carve_size<float4>(off, S);
-        carve_size<float4>(off, S);
-        carve_size<uint32_t>(off, S);
-        carve_size<uint32_t>(off, kCountSlots);
-        carve_size<JobDev>(off, NJ);
-        if (a->buf.ensure(off + 4096) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
+        if (a->buf.ensure(frame_bytes(P, S, NJ), !a->exact) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
         Carve c(a->buf.ptr);
         Frame& F = a->F;
         F.path_rgb = c.take<float4>(P);
         F.shade_a = c.take<float4>(P);
         F.shade_b = c.take<float4>(P);
-        F.vis = c.take<uint32_t>(P);
+        F.sflag = c.take<uint8_t>(S);
         F.cycles = c.take<uint32_t>(P);
         for (int q = 0; q < 2; ++q) {
             F.q[q].o = c.take<float4>(P);
@@ -1153,7 +1184,7 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
     F.side = 0;
     F.path_cap = (uint32_t)std::min<uint64_t>(a->paths_cap, 0xffffffffu);
     F.shadow_cap = (uint32_t)std::min<uint64_t>(a->shadow_cap, 0xffffffffu);
-    F.status = ctx->d_status;
+    F.status = a->d_counts + kStatusSlot;
     const unsigned G = (unsigned)trace_grid(ctx, tune);
     const unsigned Gsmall = (unsigned)std::min<uint64_t>((P.paths + 255) / 256, (uint64_t)ctx->cu_count * 16);
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, a->d_counts, ctx->d_counters, cnt);
@@ -1173,16 +1204,16 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
             if (tune.refill_closest == 0) {
                 if (side)
                     hipLaunchKernelGGL((cnt ? k_trace_closest_rounds<true, true> : k_trace_closest_rounds<false, true>),
-                                       dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, ctx->d_status);
+                                       dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, Fl.status);
                 else
                     hipLaunchKernelGGL((cnt ? k_trace_closest_rounds<true, false> : k_trace_closest_rounds<false, false>),
-                                       dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, ctx->d_status);
+                                       dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, Fl.status);
             } else {
                 auto kernel = side ? (cnt ? (single ? k_trace_closest_one<true, true> : k_trace_closest<true, true>)
                                           : (single ? k_trace_closest_one<false, true> : k_trace_closest<false, true>))
                                    : (cnt ? (single ? k_trace_closest_one<true, false> : k_trace_closest<true, false>)
                                           : (single ? k_trace_closest_one<false, false> : k_trace_closest<false, false>));
-                hipLaunchKernelGGL(kernel, dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, ctx->d_status);
+                hipLaunchKernelGGL(kernel, dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, Fl.status);
             }
         }
         {
@@ -1201,16 +1232,16 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
         if (tune.refill_occlusion == 0) {
             if (side)
                 hipLaunchKernelGGL((cnt ? k_trace_occlusion_rounds<true, true> : k_trace_occlusion_rounds<false, true>),
-                                   dim3(g), dim3(kTraceThreads), 0, st, Fo, ctx->d_status, part);
+                                   dim3(g), dim3(kTraceThreads), 0, st, Fo, Fo.status, part);
             else
                 hipLaunchKernelGGL((cnt ? k_trace_occlusion_rounds<true, false> : k_trace_occlusion_rounds<false, false>),
-                                   dim3(g), dim3(kTraceThreads), 0, st, Fo, ctx->d_status, part);
+                                   dim3(g), dim3(kTraceThreads), 0, st, Fo, Fo.status, part);
         } else if (side) {
             hipLaunchKernelGGL((cnt ? k_trace_occlusion<true, true> : k_trace_occlusion<false, true>), dim3(g),
-                               dim3(kTraceThreads), 0, st, Fo, ctx->d_status, part);
+                               dim3(kTraceThreads), 0, st, Fo, Fo.status, part);
         } else {
             hipLaunchKernelGGL((cnt ? k_trace_occlusion<true, false> : k_trace_occlusion<false, false>), dim3(g),
-                               dim3(kTraceThreads), 0, st, Fo, ctx->d_status, part);
+                               dim3(kTraceThreads), 0, st, Fo, Fo.status, part);
         }
     };
     level_step(F, 0, G, Gsmall, s, timed);
@@ -1248,11 +1279,25 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
         unsigned Gr = (unsigned)std::min<uint64_t>((P.pixels + 255) / 256, (uint64_t)ctx->cu_count * 16);
         hipLaunchKernelGGL(k_resolve, dim3(Gr), dim3(256), 0, s, F, P.pixels);
     }
+    hipLaunchKernelGGL(k_status_merge, dim3(1), dim3(64), 0, s, F.status, ctx->d_status);
 }
+
+// Marks an arena as used by this host call for its duration.
+struct ArenaUse {
+    FrameArena* a;
+    bool ok;
+    explicit ArenaUse(FrameArena* a_) : a(a_) { ok = a->users.fetch_add(1) == 0; }
+    ~ArenaUse() { a->users.fetch_sub(1); }
+};
 
 int trace_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, hipStream_t s, FrameArena* a,
                bool check_tlas, const Tuning* tune) {
     if (a == nullptr) a = arena(ctx, s);
+    ArenaUse use(a);
+    RTG_REQUIRE(use.ok, RTG_ERROR_STATE,
+                "trace: the frame state of stream %p is in use by another call (a context is used by one thread "
+                "at a time)",
+                (void*)s);
     const Tuning& t = tune ? *tune : ctx->tune;
     TracePlan P;
     int st = plan_jobs(ctx, jobs, njobs, a, check_tlas, t, P);
@@ -1346,6 +1391,11 @@ extern "C" int rtg_trace_rays(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindi
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     RTG_HIP(hipSetDevice(ctx->device));
     FrameArena* a = arena(ctx, s);
+    ArenaUse use(a);
+    RTG_REQUIRE(use.ok, RTG_ERROR_STATE,
+                "rtg_trace_rays: the frame state of stream %p is in use by another call (a context is used by one "
+                "thread at a time)",
+                (void*)s);
     if (ensure_spill(ctx, a, ctx->tune) != RTG_OK) return RTG_ERROR_OUT_OF_MEMORY;
     const unsigned G = (unsigned)std::min<uint64_t>((count + kTraceThreads - 1) / kTraceThreads, trace_grid(ctx));
     launch_ray_query(ctx, tlas, scene, rays, count, mode, ray_flags, cull_mask, sbt_offset, hits, occluded,

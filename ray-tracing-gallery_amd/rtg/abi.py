@@ -220,6 +220,10 @@ class TraceCounters(C.Structure):
         ("side_triangle_tests", C.c_uint64),
         ("side_instance_visits", C.c_uint64),
         ("side_anyhit_calls", C.c_uint64),
+        ("spill_pushes", C.c_uint64),
+        ("occ_spill_pushes", C.c_uint64),
+        ("side_spill_pushes", C.c_uint64),
+        ("side_occ_spill_pushes", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -346,3 +350,4 @@ RTG_ERROR_LIMIT = -4
 RTG_ERROR_NOT_FOUND = -5
 RTG_ERROR_BUDGET = -6
 RTG_ERROR_STATE = -7
+RTG_ERROR_RESULT = -8

@@ -50,8 +50,17 @@ def test_hip_reproduces_frame(ctx, name):
     from rtg.device import DeviceScene, hits_to_numpy
     s, a = load(name)
     ds = DeviceScene(ctx, s)
-    rgba, rgb, ph = ds.trace_image(frame_index=int(a["frame"][0]))
+    ctx.set_counters(True)
+    try:
+        rgba, rgb, ph = ds.trace_image(frame_index=int(a["frame"][0]))
+        cnt = ctx.counters()
+    finally:
+        ctx.set_counters(False)
     torch.cuda.synchronize()
+    # ray and traversal counts (stack spills included) equal the oracle's
+    for f, v in zip((f for f, _ in abi.TraceCounters._fields_), a["out_counters"]):
+        if not f.startswith("side_"):
+            assert getattr(cnt, f) == v, f
     assert np.array_equal(hits_to_numpy(ph).view(np.uint32).reshape(-1, 6), a["out_hits"])
     assert np.array_equal(rgba.cpu().numpy().view(np.uint32), a["out_rgba8"])
     err = float(np.abs(rgb.cpu().numpy() - a["out_rgb"].reshape(-1)).max())

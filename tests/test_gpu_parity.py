@@ -117,18 +117,23 @@ def test_trace_rays_small(ctx, small, mode):
 
 
 def test_trace_rays_counters_match_oracle(ctx, small):
+    """Traversal counts of both query kinds, stack pushes past the LDS part
+    (16 closest-hit, 32 occlusion entries) included, equal the oracle's."""
     s, ds, hs = small
     rays = scenes.incoherent_rays(20000, (-10, -1, -10), (10, 4, 10), seed=12)
-    ctx.set_counters(True)
-    try:
-        d = rays_to_device(rays)
-        ds.trace_rays(d, len(rays))
-        c = ctx.counters()
-    finally:
-        ctx.set_counters(False)
-    _, oc = hs.trace_rays(rays)
-    for f in ("node_visits", "triangle_tests", "instance_visits"):
-        assert getattr(c, f) == getattr(oc, f), f
+    d = rays_to_device(rays)
+    for mode, names in ((abi.RTG_QUERY_CLOSEST, ("node_visits", "triangle_tests", "instance_visits", "spill_pushes")),
+                        (abi.RTG_QUERY_OCCLUSION, ("occ_node_visits", "occ_triangle_tests", "occ_instance_visits",
+                                                   "occ_spill_pushes"))):
+        ctx.set_counters(True)
+        try:
+            ds.trace_rays(d, len(rays), mode)
+            c = ctx.counters()
+        finally:
+            ctx.set_counters(False)
+        _, oc = hs.trace_rays(rays, mode=mode)
+        for f in names:
+            assert getattr(c, f) == getattr(oc, f), f
 
 
 def _compare_image(ds, hs, frame=None, opts=None, rgb_tol=RGB_TOL):
@@ -197,10 +202,10 @@ def test_incoherent_full_scene(ctx):
             got = hits_to_numpy(out)
             assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
             assert (ref["instance"] != abi.RTG_NO_HIT).mean() > 0.5
-            names = ("node_visits", "triangle_tests", "instance_visits")
+            names = ("node_visits", "triangle_tests", "instance_visits", "spill_pushes")
         else:
             assert np.array_equal(out.cpu().numpy().astype(np.uint32), ref)
-            names = ("occ_node_visits", "occ_triangle_tests", "occ_instance_visits")
+            names = ("occ_node_visits", "occ_triangle_tests", "occ_instance_visits", "occ_spill_pushes")
         for f in names:
             assert getattr(c, f) == getattr(oc, f), f
     ds.destroy()

@@ -94,8 +94,45 @@ def test_results_alone_equal_results_in_the_mix(ctx, resident):
         load().rtg_tlas_destroy(ctx.ptr, clone)
 
 
+def spill_area_bytes(ctx):
+    """Device bytes of one slot's traversal spill area: one 640-byte stack
+    spill per lane of the persistent grid (blocks_per_cu workgroups of 256
+    lanes per CU)."""
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    return cus * ctx.tuning().blocks_per_cu * 256 * 640
+
+
+def test_budget_must_hold_every_slots_spill_area(ctx):
+    from rtg.lib import RtgError
+    spill = spill_area_bytes(ctx)
+    with pytest.raises(RtgError, match=r"\(-6\)"):
+        Server(ctx, budget_bytes=max(64 << 20, 2 * spill - (1 << 20)), max_batch=4, max_inflight=2)
+    Server(ctx, budget_bytes=2 * spill + (64 << 20), max_batch=4, max_inflight=2).close()
+
+
+def test_device_memory_stays_within_the_budget(ctx, resident):
+    spill = spill_area_bytes(ctx)
+    budget = 3 * spill + (48 << 20)
+    reqs, outs, bufs = request_set(resident, n=24, seed=9)
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    srv = Server(ctx, budget_bytes=budget, max_batch=8, max_inflight=3)
+    for ds in resident:
+        srv.add_scene(ds)
+    comp = srv.replay(reqs, np.zeros(len(reqs)))
+    torch.cuda.synchronize()
+    free1 = torch.cuda.mem_get_info()[0]
+    stats = srv.stats()
+    srv.close()
+    assert all(c.status == 0 for c in comp)
+    assert stats["max_inflight_seen"] > 1
+    # what the slots held, by the server's count and by the device's
+    assert 3 * spill <= stats["peak_bytes"] <= budget
+    assert free0 - free1 <= budget + (8 << 20)
+
+
 def test_budget_rejects_requests_that_never_fit(ctx, resident):
-    srv = Server(ctx, budget_bytes=64 << 20, max_batch=4, max_inflight=2)
+    srv = Server(ctx, budget_bytes=2 * spill_area_bytes(ctx) + (64 << 20), max_batch=4, max_inflight=2)
     srv.add_scene(resident[0])
     big = make_request(0, resident[0], 1920, 1080, 8, 4, 3, frame=0)
     st, _ = srv.submit(big)
