@@ -391,6 +391,12 @@ typedef struct rtg_trace_counters {
     uint64_t occ_spill_pushes;
     uint64_t side_spill_pushes;
     uint64_t side_occ_spill_pushes;
+    /* textured hits shaded (closest_hit_textured runs; each casts the job's
+       shadow rays), of them on the bounce stream, and the continuation
+       (bounce) rays that the bounce stream's shading adds */
+    uint64_t textured_hits;
+    uint64_t side_textured_hits;
+    uint64_t side_bounce_rays;
 } rtg_trace_counters;
 /* Enables traversal counters (slower kernels) for subsequent traces. */
 int rtg_set_counters(rtg_ctx* ctx, int enable);

@@ -418,6 +418,7 @@ static void add_counters(rtg_trace_counters* dst, const rtg_trace_counters* s) {
     dst->occ_anyhit_calls += s->occ_anyhit_calls;
     dst->spill_pushes += s->spill_pushes;
     dst->occ_spill_pushes += s->occ_spill_pushes;
+    dst->textured_hits += s->textured_hits;
 }
 
 /* ---------------------------------------------------------------- */
@@ -928,6 +929,7 @@ static v3 shade_textured(qctx* q, const rtg_image_job* job, uint32_t px, uint32_
     const rtg_instance* src = &q->tsrc[h->inst];
     uint32_t custom = src->custom_index_and_mask & 0xffffffu;
     if (custom >= q->model_count) return V3(0.0f, 0.0f, 0.0f);
+    q->cnt.textured_hits++;
     const rtg_model_info* mi = &q->models[custom];
     const rtg_geometry_info* gi = (const rtg_geometry_info*)(uintptr_t)mi->geometry_info_address + h->geom;
     const uint32_t* idx = (const uint32_t*)(uintptr_t)gi->index_buffer_address + 3ull * h->prim;

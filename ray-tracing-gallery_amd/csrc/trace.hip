@@ -162,8 +162,11 @@ constexpr int kCounterSideShadow = 11;
 constexpr int kCountersOcclusionSide = 12;
 constexpr int kCountersClosestSide = 16;
 // stack pushes past the LDS part: slots 20 (closest), 21 (occlusion) and,
-// for the bounce stream, 22 (closest) and 23 (occlusion)
-constexpr int kCounterSlots = 24;
+// for the bounce stream, 22 (closest) and 23 (occlusion); shaded textured
+// hits: slot 24 (all), 25 (bounce stream); slot 26: continuation rays the
+// bounce stream's shading adds
+constexpr int kCounterTextured = 24;
+constexpr int kCounterSlots = 27;
 
 __device__ inline int spill_slot(int base) {
     return base == kCountersClosest ? 20 : base == kCountersOcclusion ? 21 : base == kCountersClosestSide ? 22 : 23;
@@ -510,7 +513,7 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
     uint32_t* next_count = &F.counts[level + 1];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t rounds = (n + stride - 1) / stride;
-    uint32_t n_shadow = 0, n_bounce = 0;
+    uint32_t n_shadow = 0, n_bounce = 0, n_textured = 0;
     for (uint32_t r = 0; r < rounds; ++r) {
         uint32_t i = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         bool active = i < n;
@@ -696,12 +699,18 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
                 }
             }
             n_shadow += want_shadow;
+            n_textured++;
         }
     }
     if (COUNT) {
         wave_add_counter(&F.counters[1], n_bounce);
         wave_add_counter(&F.counters[2], n_shadow);
-        if (F.side) wave_add_counter(&F.counters[kCounterSideShadow], n_shadow);
+        wave_add_counter(&F.counters[kCounterTextured], n_textured);
+        if (F.side) {
+            wave_add_counter(&F.counters[kCounterSideShadow], n_shadow);
+            wave_add_counter(&F.counters[kCounterTextured + 1], n_textured);
+            wave_add_counter(&F.counters[kCounterTextured + 2], n_bounce);
+        }
     }
 }
 

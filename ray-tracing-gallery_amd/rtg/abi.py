@@ -224,6 +224,9 @@ class TraceCounters(C.Structure):
         ("occ_spill_pushes", C.c_uint64),
         ("side_spill_pushes", C.c_uint64),
         ("side_occ_spill_pushes", C.c_uint64),
+        ("textured_hits", C.c_uint64),
+        ("side_textured_hits", C.c_uint64),
+        ("side_bounce_rays", C.c_uint64),
     ]
 
     def as_dict(self):

@@ -23,7 +23,8 @@ def counters(**kw):
              "anyhit_calls", "occ_node_visits", "occ_triangle_tests", "occ_instance_visits", "occ_anyhit_calls",
              "side_shadow_rays", "side_occ_node_visits", "side_occ_triangle_tests", "side_occ_instance_visits",
              "side_occ_anyhit_calls", "side_node_visits", "side_triangle_tests", "side_instance_visits",
-             "side_anyhit_calls"]
+             "side_anyhit_calls", "spill_pushes", "occ_spill_pushes", "side_spill_pushes", "side_occ_spill_pushes",
+             "textured_hits", "side_textured_hits", "side_bounce_rays"]
     c = dict.fromkeys(names, 0)
     c.update(kw)
     return c
@@ -45,12 +46,28 @@ def test_algorithmic_bytes_of_the_frame_stream_launches():
     assert bench.algorithmic_bytes(c, "closest") == 13 * 56 + 200 * 128
 
 
-def test_measured_traffic_reads_the_committed_summary():
+def test_shade_bytes_of_the_level_0_launch():
+    # per ray 60 B in; per miss 16 B out; per textured hit 292 B; per shadow
+    # ray 44 B; per continuation 204 B; the bounce stream's share left out
+    c = counters(primary_rays=100, bounce_rays=12, shadow_rays=280, textured_hits=70, side_textured_hits=5,
+                 side_shadow_rays=20, side_bounce_rays=2, side_node_visits=1)
+    tex, shadow, cont = 65, 260, 10
+    misses = 100 - tex - cont
+    assert bench.shade_bytes(c) == 100 * 60 + misses * 16 + tex * 292 + shadow * 44 + cont * 204
+
+
+def test_measured_figures_read_the_committed_summary():
     with open(os.path.join(REPO, "profiles", "hbm_traffic_trace_kernels.json")) as f:
-        v = json.load(f)["per_launch_bytes"]["k_trace_occlusion"]
-    assert bench.measured_traffic("k_trace_occlusion") == v["fetch"] + v["write"]
-    assert bench.measured_traffic("no_such_kernel") is None
-    assert 0.0 < bench.measured_l2_hit_rate("k_trace_occlusion") < 1.0
+        d = json.load(f)
+    v = d["per_launch_bytes"]["k_trace_occlusion"]
+    traffic, hit, req = bench.measured("k_trace_occlusion")
+    assert traffic == v["fetch"] + v["write"]
+    assert 0.0 < hit < 1.0 and req == d["l2_requests_per_launch"]["k_trace_occlusion"]
+    assert bench.measured("no_such_kernel") == (None, None, None)
+    r = bench.kernel_roofline("k_trace_occlusion", 8e9, 2.0, 5)
+    assert r["achieved"] == 4000.0 and r["frac"] == 0.5
+    assert r["dram_gb_s"] == round(traffic / 2e-3 / 1e9, 1)
+    assert r["dram_frac"] == round(r["dram_gb_s"] / 8000.0, 4)
 
 
 def test_more_than_one_gpu_is_refused():
