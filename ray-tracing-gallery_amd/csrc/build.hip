@@ -448,28 +448,34 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* keys, int n, uin
     if (i == 0) parent[0] = 0xffffffffu;
 }
 
-// Bottom-up boxes of the binary tree (the second child to arrive at a node
-// finishes it) and their surface areas.
-__global__ __launch_bounds__(256) void k_bottom_up(const Box* boxes, const uint32_t* order, int n, const uint32_t* left,
-                                                   const uint32_t* right, const uint32_t* parent, uint32_t* arrivals,
-                                                   Box* ibox, float* area) {
-    int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
-    uint32_t node = parent[(n - 1) + k];
-    while (node != 0xffffffffu) {
-        __threadfence();
-        uint32_t old = atomicAdd(&arrivals[node], 1u);
-        if (old == 0) return;
-        __threadfence();
-        uint32_t c[2] = {left[node], right[node]};
-        Box cbx[2];
-#pragma unroll
-        for (int s = 0; s < 2; ++s) cbx[s] = (c[s] & kLeafBit) ? boxes[order[c[s] & kFirstMask]] : ibox[c[s]];
-        Box u = box_union(cbx[0], cbx[1]);
-        ibox[node] = u;
-        area[node] = box_area(u);
-        node = parent[node];
+// Depth of every internal node of the binary tree (the root has depth 0),
+// by walking its parent chain; the largest depth is folded into *max_depth.
+__global__ __launch_bounds__(256) void k_node_depth(int n, const uint32_t* parent, uint32_t* depth,
+                                                    uint32_t* max_depth) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    uint32_t d = 0;
+    if (i < n - 1) {
+        for (uint32_t p = parent[i]; p != 0xffffffffu; p = parent[p]) ++d;
+        depth[i] = d;
     }
+    for (int off = 32; off > 0; off >>= 1) d = max(d, (uint32_t)__shfl_xor((int)d, off));
+    if ((threadIdx.x & 63) == 0 && d) atomicMax(max_depth, d);
+}
+
+// Boxes and surface areas of the internal nodes at depth `d` (the deeper
+// ones are done).
+__global__ __launch_bounds__(256) void k_boxes_level(uint32_t d, const uint32_t* depth, const Box* boxes,
+                                                     const uint32_t* order, int n, const uint32_t* left,
+                                                     const uint32_t* right, Box* ibox, float* area) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n - 1 || depth[i] != d) return;
+    uint32_t c[2] = {left[i], right[i]};
+    Box cbx[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) cbx[s] = (c[s] & kLeafBit) ? boxes[order[c[s] & kFirstMask]] : ibox[c[s]];
+    Box u = box_union(cbx[0], cbx[1]);
+    ibox[i] = u;
+    area[i] = box_area(u);
 }
 
 // ---------------------------------------------------------------------------
@@ -496,8 +502,32 @@ __device__ inline Box bin_box(const Box* boxes, const uint32_t* order, const Box
     return (c & kLeafBit) ? boxes[order[c & kFirstMask]] : ibox[c];
 }
 
+// The cheapest split of leaf set S: the partition (P, S \ P) with P holding
+// S's lowest leaf that minimises cost(P) + cost(S \ P), partitions tried in
+// decreasing order of P and the first minimum kept. P = low | Q for Q a
+// proper submask of S \ low, so decreasing Q is decreasing P.
+__device__ inline uint32_t best_split(uint32_t S, const float* cost, int stride, float& best) {
+    const uint32_t low = S & (0u - S), R = S ^ low;
+    uint32_t bp = 0;
+    best = 0.0f;
+    for (uint32_t Q = (R - 1u) & R;; Q = (Q - 1u) & R) {
+        const uint32_t P = Q | low;
+        const float c = cost[P * stride] + cost[(S ^ P) * stride];
+        if (bp == 0 || c < best) {
+            best = c;
+            bp = P;
+        }
+        if (Q == 0) break;
+    }
+    return bp;
+}
+
+// `cost` is this lane's column of a per-workgroup LDS table (entry S at
+// cost[S * stride]); LDS keeps the dynamic programme's 2^7 subset costs
+// close, where a per-lane array would live in scratch memory.
 __device__ void treelet_optimize(uint32_t r, const Box* boxes, const uint32_t* order, uint32_t* left,
-                                 uint32_t* right, uint32_t* rcount, Box* ibox, float* area) {
+                                 uint32_t* right, uint32_t* rcount, Box* ibox, float* area, float* cost,
+                                 int stride) {
     uint32_t leaf[kTreeletLeaves], inner[kTreeletLeaves];
     int nl = 2, ni = 0;
     leaf[0] = left[r];
@@ -521,33 +551,24 @@ __device__ void treelet_optimize(uint32_t r, const Box* boxes, const uint32_t* o
         leaf[nl++] = right[c];
     }
     if (nl < 3) return;
+    // leaf boxes in registers: the loops over them are unrolled to constant
+    // indices (set bits of S beyond nl never occur)
     Box lb[kTreeletLeaves];
-    for (int k = 0; k < nl; ++k) lb[k] = bin_box(boxes, order, ibox, leaf[k]);
+#pragma unroll
+    for (int k = 0; k < kTreeletLeaves; ++k) lb[k] = k < nl ? bin_box(boxes, order, ibox, leaf[k]) : box_empty();
     const uint32_t full = (1u << nl) - 1u;
-    float cost[1 << kTreeletLeaves];
-    uint8_t split[1 << kTreeletLeaves];
     for (uint32_t S = 1; S <= full; ++S) {
         if ((S & (S - 1u)) == 0) {
-            cost[S] = 0.0f;
-            split[S] = 0;
+            cost[S * stride] = 0.0f;
             continue;
         }
         Box b = box_empty();
-        for (int k = 0; k < nl; ++k)
+#pragma unroll
+        for (int k = 0; k < kTreeletLeaves; ++k)
             if (S & (1u << k)) b = box_union(b, lb[k]);
-        const uint32_t low = S & (0u - S);
-        float best = 0.0f;
-        uint32_t bp = 0;
-        for (uint32_t P = (S - 1u) & S; P != 0; P = (P - 1u) & S) {
-            if (!(P & low)) continue;
-            const float c = cost[P] + cost[S ^ P];
-            if (bp == 0 || c < best) {
-                best = c;
-                bp = P;
-            }
-        }
-        cost[S] = box_area(b) + best;
-        split[S] = (uint8_t)bp;
+        float best;
+        best_split(S, cost, stride, best);
+        cost[S * stride] = box_area(b) + best;
     }
     uint32_t stk_set[kTreeletLeaves], stk_id[kTreeletLeaves], done[kTreeletLeaves];
     int sp = 0, no = 0, next = 0;
@@ -557,7 +578,9 @@ __device__ void treelet_optimize(uint32_t r, const Box* boxes, const uint32_t* o
         --sp;
         const uint32_t S = stk_set[sp], id = stk_id[sp];
         done[no++] = id;
-        const uint32_t part[2] = {split[S], S ^ split[S]};
+        float unused;
+        const uint32_t P = best_split(S, cost, stride, unused);
+        const uint32_t part[2] = {P, S ^ P};
         uint32_t ref[2];
         for (int h = 0; h < 2; ++h)
             ref[h] = (part[h] & (part[h] - 1u)) ? inner[next++] : leaf[__builtin_ctz(part[h])];
@@ -578,21 +601,22 @@ __device__ void treelet_optimize(uint32_t r, const Box* boxes, const uint32_t* o
     }
 }
 
-// One pass: every internal node restructured after its subtree.
-__global__ __launch_bounds__(256) void k_treelet(const Box* boxes, const uint32_t* order, int n, uint32_t* left,
-                                                 uint32_t* right, const uint32_t* parent, uint32_t* arrivals,
-                                                 uint32_t* rcount, Box* ibox, float* area) {
-    int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
-    uint32_t node = parent[(n - 1) + k];
-    while (node != 0xffffffffu) {
-        __threadfence();
-        uint32_t old = atomicAdd(&arrivals[node], 1u);
-        if (old == 0) return;
-        __threadfence();
-        treelet_optimize(node, boxes, order, left, right, rcount, ibox, area);
-        node = parent[node];
-    }
+// One depth level of a pass: every internal node at depth `d` of the tree
+// the pass started from. Launched from the deepest level up, so a node is
+// restructured after its whole subtree; nodes of one level have disjoint
+// subtrees, and a restructured subtree keeps its node set, so the result is
+// the one of any order that visits subtrees first.
+constexpr int kTreeletThreads = 64;
+
+__global__ __launch_bounds__(kTreeletThreads) void k_treelet_level(uint32_t d, const uint32_t* depth,
+                                                                   const Box* boxes, const uint32_t* order, int n,
+                                                                   uint32_t* left, uint32_t* right, uint32_t* rcount,
+                                                                   Box* ibox, float* area) {
+    __shared__ float s_cost[(1 << kTreeletLeaves) * kTreeletThreads];
+    int i = blockIdx.x * kTreeletThreads + threadIdx.x;
+    if (i >= n - 1 || depth[i] != d) return;
+    treelet_optimize((uint32_t)i, boxes, order, left, right, rcount, ibox, area, s_cost + threadIdx.x,
+                     kTreeletThreads);
 }
 
 // Parents of the restructured tree (the root keeps none).
@@ -852,7 +876,7 @@ static size_t lbvh_scratch_bytes(uint32_t n, size_t rec_bytes) {
     carve_size<uint32_t>(off, n);                     // left
     carve_size<uint32_t>(off, n);                     // right
     carve_size<uint32_t>(off, 2ull * n);              // parent
-    carve_size<uint32_t>(off, n);                     // arrivals
+    carve_size<uint32_t>(off, n);                     // arrivals (node depths)
     carve_size<uint32_t>(off, n);                     // range first
     carve_size<uint32_t>(off, n);                     // range count
     carve_size<float>(off, n);                        // binary box areas
@@ -937,15 +961,38 @@ static int wide_hierarchy(LbvhBufs& b, uint32_t n, uint32_t leaf_max, int treele
     // after 4 passes the sorted data is back in (ka, va)
     hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, b.ka, (int)n, b.left, b.right,
                        b.parent, b.rfirst, b.rcount);
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.arrivals, (uint64_t)n, 0u);
-    hipLaunchKernelGGL(k_bottom_up, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.boxes, b.va, (int)n, b.left, b.right,
-                       b.parent, b.arrivals, b.ibox, b.area);
+    // Bottom-up passes go one depth level per launch, deepest first: the
+    // kernel boundaries order the levels, where arrival counters would need
+    // device-scope atomics and fences, which MI355X serves past its per-XCD
+    // L2s. `b.arrivals` holds the depths; the host reads the tree's depth.
+    uint32_t* depth = b.arrivals;
+    uint32_t* max_depth = b.next + 1;
+    auto tree_depth = [&](uint32_t& out) -> int {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, max_depth, 1ull, 0u);
+        hipLaunchKernelGGL(k_node_depth, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, (int)n, b.parent, depth,
+                           max_depth);
+        RTG_HIP(hipGetLastError());
+        RTG_HIP(hipMemcpyAsync(&out, max_depth, 4, hipMemcpyDeviceToHost, s));
+        RTG_HIP(hipStreamSynchronize(s));
+        RTG_REQUIRE(out < n, RTG_ERROR_DEVICE, "BVH build: tree depth %u over %u primitives", out, n);
+        return RTG_OK;
+    };
+    uint32_t dmax = 0;
+    int st = tree_depth(dmax);
+    if (st != RTG_OK) return st;
+    for (uint32_t d = dmax + 1; d-- > 0;)
+        hipLaunchKernelGGL(k_boxes_level, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, d, depth, b.boxes, b.va,
+                           (int)n, b.left, b.right, b.ibox, b.area);
     // the treelet passes need single-primitive leaves: a restructured
     // subtree no longer covers a contiguous range of sorted primitives
     for (int round = 0; round < treelet_rounds && leaf_max == 1; ++round) {
-        hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.arrivals, (uint64_t)n, 0u);
-        hipLaunchKernelGGL(k_treelet, dim3(blocks_for(n, 256)), dim3(256), 0, s, b.boxes, b.va, (int)n, b.left,
-                           b.right, b.parent, b.arrivals, b.rcount, b.ibox, b.area);
+        if (round > 0) {
+            st = tree_depth(dmax);
+            if (st != RTG_OK) return st;
+        }
+        for (uint32_t d = dmax + 1; d-- > 0;)
+            hipLaunchKernelGGL(k_treelet_level, dim3(blocks_for(n - 1, kTreeletThreads)), dim3(kTreeletThreads), 0, s,
+                               d, depth, b.boxes, b.va, (int)n, b.left, b.right, b.rcount, b.ibox, b.area);
         hipLaunchKernelGGL(k_parents, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, (int)n, b.left, b.right,
                            b.parent);
     }

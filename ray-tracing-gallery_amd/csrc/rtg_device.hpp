@@ -661,82 +661,90 @@ struct Traverser {
         return sp1 > from ? (uint32_t)(sp1 - from) : 0u;
     }
 
-    __device__ inline int step(const TlasView& T, const SceneView& S, StackT& st, TravCounters& cnt) {
-        if (!(cur & kLeafBit)) {
-            const RayCtx& cr = in_blas ? obj : world;
-            const float4* np = reinterpret_cast<const float4*>(nodes + cur);
-            float4 LX = gld_f4(np), HX = gld_f4(np + 1), LY = gld_f4(np + 2), HY = gld_f4(np + 3);
-            float4 LZ = gld_f4(np + 4), HZ = gld_f4(np + 5);
-            uint4 C = gld_u4(np + 6);
-            if (COUNT) cnt.nodes++;
-            const bool sx = signbit(cr.inv.x), sy = signbit(cr.inv.y), sz = signbit(cr.inv.z);
-            float t0, t1, t2, t3;
-            bool h0 = slab(LX.x, HX.x, LY.x, HY.x, LZ.x, HZ.x, cr, sx, sy, sz, tmin, tcur, t0) && C.x != kEmpty;
-            bool h1 = slab(LX.y, HX.y, LY.y, HY.y, LZ.y, HZ.y, cr, sx, sy, sz, tmin, tcur, t1) && C.y != kEmpty;
-            bool h2 = slab(LX.z, HX.z, LY.z, HY.z, LZ.z, HZ.z, cr, sx, sy, sz, tmin, tcur, t2) && C.z != kEmpty;
-            bool h3 = slab(LX.w, HX.w, LY.w, HY.w, LZ.w, HZ.w, cr, sx, sy, sz, tmin, tcur, t3) && C.w != kEmpty;
-            const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
-            if (nh == 0) {
-                if (!pop(T, st)) return done();
-                return kRayActive;
-            }
-            if (st.sp + nh - 1 > kStackMax) return kRayOverflow;
-            if (COUNT) cnt.spills += spilled(st.sp, st.sp + nh - 1);
-            if (OCCLUSION) {
-                // slot order: the lowest overlapped slot is visited next, the
-                // others are pushed highest first
-                if (h3 && (h0 || h1 || h2)) st.push(C.w, t3);
-                if (h2 && (h0 || h1)) st.push(C.z, t2);
-                if (h1 && h0) st.push(C.y, t1);
-                cur = h0 ? C.x : h1 ? C.y : h2 ? C.z : C.w;
-                return kRayActive;
-            }
-            // rank of each overlapped slot in ascending (entry distance, slot)
-            // order; the rank-0 slot is visited next, ranks nh-1 .. 1 are
-            // pushed. A rank held by none of slots 0-2 is slot 3's.
-            const int r0 = (h1 && t1 < t0) + (h2 && t2 < t0) + (h3 && t3 < t0);
-            const int r1 = (h0 && t0 <= t1) + (h2 && t2 < t1) + (h3 && t3 < t1);
-            const int r2 = (h0 && t0 <= t2) + (h1 && t1 <= t2) + (h3 && t3 < t2);
+    // An internal node: slab tests of its four slots; the nearest overlapped
+    // child (the first in slot order for occlusion queries) is next, the
+    // others are pushed.
+    __device__ inline int node_step(const TlasView& T, StackT& st, TravCounters& cnt) {
+        const RayCtx& cr = in_blas ? obj : world;
+        const float4* np = reinterpret_cast<const float4*>(nodes + cur);
+        float4 LX = gld_f4(np), HX = gld_f4(np + 1), LY = gld_f4(np + 2), HY = gld_f4(np + 3);
+        float4 LZ = gld_f4(np + 4), HZ = gld_f4(np + 5);
+        uint4 C = gld_u4(np + 6);
+        if (COUNT) cnt.nodes++;
+        const bool sx = signbit(cr.inv.x), sy = signbit(cr.inv.y), sz = signbit(cr.inv.z);
+        float t0, t1, t2, t3;
+        bool h0 = slab(LX.x, HX.x, LY.x, HY.x, LZ.x, HZ.x, cr, sx, sy, sz, tmin, tcur, t0) && C.x != kEmpty;
+        bool h1 = slab(LX.y, HX.y, LY.y, HY.y, LZ.y, HZ.y, cr, sx, sy, sz, tmin, tcur, t1) && C.y != kEmpty;
+        bool h2 = slab(LX.z, HX.z, LY.z, HY.z, LZ.z, HZ.z, cr, sx, sy, sz, tmin, tcur, t2) && C.z != kEmpty;
+        bool h3 = slab(LX.w, HX.w, LY.w, HY.w, LZ.w, HZ.w, cr, sx, sy, sz, tmin, tcur, t3) && C.w != kEmpty;
+        const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+        if (nh == 0) {
+            if (!pop(T, st)) return done();
+            return kRayActive;
+        }
+        if (st.sp + nh - 1 > kStackMax) return kRayOverflow;
+        if (COUNT) cnt.spills += spilled(st.sp, st.sp + nh - 1);
+        if (OCCLUSION) {
+            // slot order: the lowest overlapped slot is visited next, the
+            // others are pushed highest first
+            if (h3 && (h0 || h1 || h2)) st.push(C.w, t3);
+            if (h2 && (h0 || h1)) st.push(C.z, t2);
+            if (h1 && h0) st.push(C.y, t1);
+            cur = h0 ? C.x : h1 ? C.y : h2 ? C.z : C.w;
+            return kRayActive;
+        }
+        // rank of each overlapped slot in ascending (entry distance, slot)
+        // order; the rank-0 slot is visited next, ranks nh-1 .. 1 are
+        // pushed. A rank held by none of slots 0-2 is slot 3's.
+        const int r0 = (h1 && t1 < t0) + (h2 && t2 < t0) + (h3 && t3 < t0);
+        const int r1 = (h0 && t0 <= t1) + (h2 && t2 < t1) + (h3 && t3 < t1);
+        const int r2 = (h0 && t0 <= t2) + (h1 && t1 <= t2) + (h3 && t3 < t2);
 #pragma unroll
-            for (int q = 3; q >= 1; --q) {
-                if (q < nh) {
-                    uint32_t ref = (h0 && r0 == q) ? C.x : (h1 && r1 == q) ? C.y : (h2 && r2 == q) ? C.z : C.w;
-                    float tq = (h0 && r0 == q) ? t0 : (h1 && r1 == q) ? t1 : (h2 && r2 == q) ? t2 : t3;
-                    st.push(ref, tq);
-                }
+        for (int q = 3; q >= 1; --q) {
+            if (q < nh) {
+                uint32_t ref = (h0 && r0 == q) ? C.x : (h1 && r1 == q) ? C.y : (h2 && r2 == q) ? C.z : C.w;
+                float tq = (h0 && r0 == q) ? t0 : (h1 && r1 == q) ? t1 : (h2 && r2 == q) ? t2 : t3;
+                st.push(ref, tq);
             }
-            cur = (h0 && r0 == 0) ? C.x : (h1 && r1 == 0) ? C.y : (h2 && r2 == 0) ? C.z : C.w;
+        }
+        cur = (h0 && r0 == 0) ? C.x : (h1 && r1 == 0) ? C.y : (h2 && r2 == 0) ? C.z : C.w;
+        return kRayActive;
+    }
+
+    // A TLAS leaf: enter its instance's BLAS (the object-space ray, a
+    // sentinel on the stack) or, if the cull mask rejects it, skip it.
+    __device__ inline int instance_step(const TlasView& T, StackT& st, TravCounters& cnt) {
+        const InstRec* rec = T.recs + (cur & kFirstMask);
+        const float4* rp = reinterpret_cast<const float4*>(rec);
+        uint4 tail = gld_u4(rp + 3);  // w2o[8..11] is rp[2]; rp[3] = blas lo, blas hi, index, mask_flags
+        uint32_t mf = tail.w;
+        const uint8_t* bb = reinterpret_cast<const uint8_t*>((uint64_t)tail.x | ((uint64_t)tail.y << 32));
+        if (((mf & 0xffu) & cull_mask) == 0 || bb == nullptr) {
+            if (!pop(T, st)) return done();
             return kRayActive;
         }
-        if (!in_blas) {
-            const InstRec* rec = T.recs + (cur & kFirstMask);
-            const float4* rp = reinterpret_cast<const float4*>(rec);
-            uint4 tail = gld_u4(rp + 3);  // w2o[8..11] is rp[2]; rp[3] = blas lo, blas hi, index, mask_flags
-            uint32_t mf = tail.w;
-            const uint8_t* bb = reinterpret_cast<const uint8_t*>((uint64_t)tail.x | ((uint64_t)tail.y << 32));
-            if (((mf & 0xffu) & cull_mask) == 0 || bb == nullptr) {
-                if (!pop(T, st)) return done();
-                return kRayActive;
-            }
-            if (COUNT) cnt.insts++;
-            if (st.sp >= kStackMax) return kRayOverflow;
-            if (COUNT) cnt.spills += spilled(st.sp, st.sp + 1);
-            st.push(kSentinel, -__builtin_inff());
-            float w2o[12];
-            float4 m0 = gld_f4(rp), m1 = gld_f4(rp + 1), m2 = gld_f4(rp + 2);
-            w2o[0] = m0.x; w2o[1] = m0.y; w2o[2] = m0.z; w2o[3] = m0.w;
-            w2o[4] = m1.x; w2o[5] = m1.y; w2o[6] = m1.z; w2o[7] = m1.w;
-            w2o[8] = m2.x; w2o[9] = m2.y; w2o[10] = m2.z; w2o[11] = m2.w;
-            ray_setup(obj, m34_point(w2o, world.o), m34_dir(w2o, world.d));
-            const AsHeader* bh = reinterpret_cast<const AsHeader*>(bb);
-            nodes = reinterpret_cast<const Node*>(bb + gld_u64(&bh->nodes_off));
-            tris = reinterpret_cast<const TriRec*>(bb + gld_u64(&bh->prims_off));
-            inst_index = tail.z;
-            inst_flags = (mf >> 8) & 0xffu;
-            in_blas = true;
-            cur = 0;
-            return kRayActive;
-        }
+        if (COUNT) cnt.insts++;
+        if (st.sp >= kStackMax) return kRayOverflow;
+        if (COUNT) cnt.spills += spilled(st.sp, st.sp + 1);
+        st.push(kSentinel, -__builtin_inff());
+        float w2o[12];
+        float4 m0 = gld_f4(rp), m1 = gld_f4(rp + 1), m2 = gld_f4(rp + 2);
+        w2o[0] = m0.x; w2o[1] = m0.y; w2o[2] = m0.z; w2o[3] = m0.w;
+        w2o[4] = m1.x; w2o[5] = m1.y; w2o[6] = m1.z; w2o[7] = m1.w;
+        w2o[8] = m2.x; w2o[9] = m2.y; w2o[10] = m2.z; w2o[11] = m2.w;
+        ray_setup(obj, m34_point(w2o, world.o), m34_dir(w2o, world.d));
+        const AsHeader* bh = reinterpret_cast<const AsHeader*>(bb);
+        nodes = reinterpret_cast<const Node*>(bb + gld_u64(&bh->nodes_off));
+        tris = reinterpret_cast<const TriRec*>(bb + gld_u64(&bh->prims_off));
+        inst_index = tail.z;
+        inst_flags = (mf >> 8) & 0xffu;
+        in_blas = true;
+        cur = 0;
+        return kRayActive;
+    }
+
+    // A BLAS leaf: its triangles, then the next stack entry.
+    __device__ inline int leaf_step(const TlasView& T, const SceneView& S, StackT& st, TravCounters& cnt) {
         uint32_t first = cur & kFirstMask;
         uint32_t count = ((cur >> 27) & 0xfu) + 1u;
         for (uint32_t k = first; k < first + count; ++k) {
@@ -768,6 +776,25 @@ struct Traverser {
             tcur = t;
         }
         if (!pop(T, st)) return done();
+        return kRayActive;
+    }
+
+    // One iteration: a BLAS leaf, then a TLAS leaf, then an internal node,
+    // each if it is what the ray has to visit at that point. A ray thus
+    // tests a leaf and enters an instance or visits the node that follows in
+    // the same iteration, so a wavefront's lanes, which need different kinds
+    // of visits at once, each advance further per iteration. The visits a ray
+    // makes and their order are those of one visit per iteration.
+    __device__ inline int step(const TlasView& T, const SceneView& S, StackT& st, TravCounters& cnt) {
+        if ((cur & kLeafBit) && in_blas) {
+            const int r = leaf_step(T, S, st, cnt);
+            if (r != kRayActive) return r;
+        }
+        if ((cur & kLeafBit) && !in_blas) {
+            const int r = instance_step(T, st, cnt);
+            if (r != kRayActive) return r;
+        }
+        if (!(cur & kLeafBit)) return node_step(T, st, cnt);
         return kRayActive;
     }
 };

@@ -55,13 +55,6 @@ __device__ inline void order_to_pixel(uint32_t t, uint32_t W, uint32_t H, uint32
         y = full + i % h;
     }
 }
-__device__ inline uint32_t pixel_to_order(uint32_t x, uint32_t y, uint32_t W, uint32_t H, uint32_t order) {
-    if (order == 0) return y * W + x;
-    uint32_t full = (H / 8) * 8;
-    if (y < full) return (y / 8) * 8 * W + x * 8 + (y & 7);
-    uint32_t h = H - full;
-    return full * W + x * h + (y - full);
-}
 
 struct JobDev {
     const uint8_t* tlas;
@@ -861,9 +854,13 @@ __device__ inline V3 heatmap_temperature(float heat) {
 __global__ __launch_bounds__(256) void k_resolve(Frame F, uint64_t total_pixels) {
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < total_pixels; q += stride) {
+        // lanes take pixels in path order, so that they read consecutive
+        // paths' state; the output pixel follows from the order
         const JobDev& J = F.jobs[find_job(F.jobs, F.njobs, q, true)];
-        uint32_t pix = (uint32_t)(q - J.pixel_base);
-        uint32_t t = pixel_to_order(pix % J.width, pix / J.width, J.width, J.height, F.order);
+        const uint32_t t = (uint32_t)(q - J.pixel_base);
+        uint32_t px, py;
+        order_to_pixel(t, J.width, J.height, px, py, F.order);
+        const uint32_t pix = py * J.width + px;
         V3 acc = v3(0.0f, 0.0f, 0.0f);
         uint32_t cycles = 0;
         for (uint32_t s = 0; s < J.spp; ++s) {
