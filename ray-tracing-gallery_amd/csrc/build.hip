@@ -505,19 +505,40 @@ __device__ inline Box bin_box(const Box* boxes, const uint32_t* order, const Box
 // The cheapest split of leaf set S: the partition (P, S \ P) with P holding
 // S's lowest leaf that minimises cost(P) + cost(S \ P), partitions tried in
 // decreasing order of P and the first minimum kept. P = low | Q for Q a
-// proper submask of S \ low, so decreasing Q is decreasing P.
+// proper submask of S \ low (2^m - 1 of them for m bits), and decreasing Q is
+// decreasing P. Four partitions at a time, so that their LDS reads are in
+// flight together; the comparisons stay in order.
 __device__ inline uint32_t best_split(uint32_t S, const float* cost, int stride, float& best) {
     const uint32_t low = S & (0u - S), R = S ^ low;
+    const uint32_t count = (1u << __builtin_popcount(R)) - 1u;
+    uint32_t Q = (R - 1u) & R;
     uint32_t bp = 0;
     best = 0.0f;
-    for (uint32_t Q = (R - 1u) & R;; Q = (Q - 1u) & R) {
+    uint32_t i = 0;
+    for (; i + 4u <= count; i += 4u) {
+        uint32_t P[4];
+        float c[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            P[j] = Q | low;
+            c[j] = cost[P[j] * stride] + cost[(S ^ P[j]) * stride];
+            Q = (Q - 1u) & R;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (bp == 0 || c[j] < best) {
+                best = c[j];
+                bp = P[j];
+            }
+    }
+    for (; i < count; ++i) {
         const uint32_t P = Q | low;
         const float c = cost[P * stride] + cost[(S ^ P) * stride];
         if (bp == 0 || c < best) {
             best = c;
             bp = P;
         }
-        if (Q == 0) break;
+        Q = (Q - 1u) & R;
     }
     return bp;
 }
@@ -557,7 +578,10 @@ __device__ void treelet_optimize(uint32_t r, const Box* boxes, const uint32_t* o
 #pragma unroll
     for (int k = 0; k < kTreeletLeaves; ++k) lb[k] = k < nl ? bin_box(boxes, order, ibox, leaf[k]) : box_empty();
     const uint32_t full = (1u << nl) - 1u;
-    for (uint32_t S = 1; S <= full; ++S) {
+    for (uint32_t Sl = 1; Sl <= full; ++Sl) {
+        // lanes go through the sets in step (lanes past their `full` have
+        // left the loop), so S is uniform: its table reads are scalar
+        const uint32_t S = (uint32_t)__builtin_amdgcn_readfirstlane((int)Sl);
         if ((S & (S - 1u)) == 0) {
             cost[S * stride] = 0.0f;
             continue;
@@ -608,15 +632,34 @@ __device__ void treelet_optimize(uint32_t r, const Box* boxes, const uint32_t* o
 // the one of any order that visits subtrees first.
 constexpr int kTreeletThreads = 64;
 
-__global__ __launch_bounds__(kTreeletThreads) void k_treelet_level(uint32_t d, const uint32_t* depth,
-                                                                   const Box* boxes, const uint32_t* order, int n,
+// Restructures the `count` internal nodes of one depth level listed in
+// `nodes` (the level's run of the nodes sorted by depth), so that every
+// workgroup is full of treelets.
+__global__ __launch_bounds__(kTreeletThreads) void k_treelet_level(const uint32_t* nodes, uint32_t count,
+                                                                   const Box* boxes, const uint32_t* order,
                                                                    uint32_t* left, uint32_t* right, uint32_t* rcount,
                                                                    Box* ibox, float* area) {
     __shared__ float s_cost[(1 << kTreeletLeaves) * kTreeletThreads];
-    int i = blockIdx.x * kTreeletThreads + threadIdx.x;
-    if (i >= n - 1 || depth[i] != d) return;
-    treelet_optimize((uint32_t)i, boxes, order, left, right, rcount, ibox, area, s_cost + threadIdx.x,
+    uint32_t k = blockIdx.x * kTreeletThreads + threadIdx.x;
+    if (k >= count) return;
+    treelet_optimize(nodes[k], boxes, order, left, right, rcount, ibox, area, s_cost + threadIdx.x,
                      kTreeletThreads);
+}
+
+__global__ __launch_bounds__(256) void k_iota(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+// First position of each depth in the depth-sorted node list: where the
+// depth differs from its predecessor's. Every depth from 0 to the tree's
+// depth holds a node (the ancestors of a deepest one); the entry past the
+// deepest is the list length, filled beforehand.
+__global__ __launch_bounds__(256) void k_level_starts(const uint32_t* sorted_depth, uint32_t m, uint32_t* start) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t d = sorted_depth[i];
+    if (i == 0 || sorted_depth[i - 1] != d) start[d] = i;
 }
 
 // Parents of the restructured tree (the root keeps none).
@@ -885,7 +928,7 @@ static size_t lbvh_scratch_bytes(uint32_t n, size_t rec_bytes) {
     carve_size<uint32_t>(off, cap);                   // binary root of each wide node
     carve_size<uint4>(off, cap);                      // slots of a level
     carve_size<uint32_t>(off, cap);                   // opened slots per node / prefix sums
-    carve_size<uint32_t>(off, 4);                     // next level size
+    carve_size<uint32_t>(off, 4 + 257);               // next level size, tree depth, level starts
     carve_size<GeomDesc>(off, 1);
     return off + 4096;
 }
@@ -929,7 +972,7 @@ static LbvhBufs carve_lbvh(void* base, uint32_t n, size_t rec_bytes, uint32_t ng
     b.wroot = c.take<uint32_t>(cap);
     b.slots = c.take<uint4>(cap);
     b.cnt = c.take<uint32_t>(cap);
-    b.next = c.take<uint32_t>(4);
+    b.next = c.take<uint32_t>(4 + 257);
     return b;
 }
 
@@ -984,15 +1027,39 @@ static int wide_hierarchy(LbvhBufs& b, uint32_t n, uint32_t leaf_max, int treele
         hipLaunchKernelGGL(k_boxes_level, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, d, depth, b.boxes, b.va,
                            (int)n, b.left, b.right, b.ibox, b.area);
     // the treelet passes need single-primitive leaves: a restructured
-    // subtree no longer covers a contiguous range of sorted primitives
+    // subtree no longer covers a contiguous range of sorted primitives.
+    // Each pass sorts the internal nodes by depth (one 8-bit radix pass:
+    // depth values in b.arrivals, node numbers from b.ka, which the Morton
+    // codes no longer need, into b.kb / b.vb) and restructures each depth's
+    // run with full workgroups, deepest first.
+    const uint32_t m = n - 1;
+    const uint32_t mtiles = (m + kRsTile - 1) / kRsTile;
+    std::vector<uint32_t> start;
     for (int round = 0; round < treelet_rounds && leaf_max == 1; ++round) {
         if (round > 0) {
             st = tree_depth(dmax);
             if (st != RTG_OK) return st;
         }
-        for (uint32_t d = dmax + 1; d-- > 0;)
-            hipLaunchKernelGGL(k_treelet_level, dim3(blocks_for(n - 1, kTreeletThreads)), dim3(kTreeletThreads), 0, s,
-                               d, depth, b.boxes, b.va, (int)n, b.left, b.right, b.rcount, b.ibox, b.area);
+        RTG_REQUIRE(dmax < 256u, RTG_ERROR_LIMIT, "BVH build: binary tree depth %u beyond 255", dmax);
+        hipLaunchKernelGGL(k_iota, dim3(blocks_for(m, 256)), dim3(256), 0, s, b.ka, m);
+        hipLaunchKernelGGL(k_rs_hist, dim3(mtiles), dim3(kRsThreads), 0, s, depth, m, 0, b.hist, mtiles);
+        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, b.hist, 256u * mtiles);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(mtiles), dim3(kRsThreads), 0, s, depth, b.ka, b.kb, b.vb, m, 0, b.hist,
+                           mtiles);
+        uint32_t* d_start = b.next + 4;  // 257 words after the collapse's counters
+        hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(dmax + 2, 256)), dim3(256), 0, s, d_start,
+                           (uint64_t)(dmax + 2), m);
+        hipLaunchKernelGGL(k_level_starts, dim3(blocks_for(m, 256)), dim3(256), 0, s, b.kb, m, d_start);
+        RTG_HIP(hipGetLastError());
+        start.assign(dmax + 2, m);
+        RTG_HIP(hipMemcpyAsync(start.data(), d_start, (dmax + 2) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        RTG_HIP(hipStreamSynchronize(s));
+        for (uint32_t d = dmax + 1; d-- > 0;) {
+            const uint32_t count = start[d + 1] - start[d];
+            if (count == 0) continue;
+            hipLaunchKernelGGL(k_treelet_level, dim3(blocks_for(count, kTreeletThreads)), dim3(kTreeletThreads), 0, s,
+                               b.vb + start[d], count, b.boxes, b.va, b.left, b.right, b.rcount, b.ibox, b.area);
+        }
         hipLaunchKernelGGL(k_parents, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, (int)n, b.left, b.right,
                            b.parent);
     }
