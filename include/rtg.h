@@ -308,7 +308,7 @@ int rtg_trace_rays(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindings* scene,
  * GPU shares that GPU's array: calls that overlap on one GPU (two streams,
  * two threads, two contexts) race and must be serialised by the caller. */
 #define RTG_STATIC_QUERY_BLOCKS 512u
-#define RTG_STATIC_SCRATCH_BYTES 83886336u /* 256 + 512 * 256 lanes * 640 B of spill stack */
+#define RTG_STATIC_SCRATCH_BYTES 92274944u /* 256 + 512 * 256 lanes * 704 B of spill stack */
 int rtg_trace_rays_static(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindings* scene,
                           const rtg_ray* rays, uint32_t count, uint32_t mode, uint32_t ray_flags,
                           uint32_t cull_mask, uint32_t sbt_offset, rtg_hit* hits, uint32_t* occluded,
@@ -465,7 +465,11 @@ typedef struct rtg_tuning {
                                   shadow rays they add, on a second stream beside the first level's
                                   shadow rays (taken from the blocks_per_cu grid, at most half of it);
                                   0 = every level in sequence on the caller's stream */
-    uint32_t reserved[2];      /* zero */
+    uint32_t stack_lds_entries; /* traversal stack entries per lane kept in LDS (8..32, multiple of 4;
+                                  0 = 16): a workgroup takes entries * 2 KB of LDS, deeper entries
+                                  spill to global memory; occlusion stacks keep twice as many
+                                  (half-size) entries */
+    uint32_t reserved;         /* zero */
 } rtg_tuning;
 int rtg_get_tuning(rtg_ctx* ctx, rtg_tuning* out);
 int rtg_set_tuning(rtg_ctx* ctx, const rtg_tuning* tuning);
@@ -510,7 +514,7 @@ typedef struct rtg_server rtg_server;
 
 typedef struct rtg_server_config {
     uint64_t hbm_budget_bytes;  /* device memory of all slots: each of the max_inflight slots holds the spill area
-                                   of the persistent traversal grid (cu_count * blocks_per_cu * 256 lanes * 640 B)
+                                   of the persistent traversal grid (cu_count * blocks_per_cu * 256 lanes * 704 B)
                                    and the frame state and TLAS copies of its batches within its share
                                    (budget / max_inflight); rtg_server_create refuses (RTG_ERROR_BUDGET) a budget
                                    whose share cannot hold a spill area */

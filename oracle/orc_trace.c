@@ -248,6 +248,10 @@ static int key_less(uint32_t ia, uint32_t ga, uint32_t pa, uint32_t ib, uint32_t
     return pa < pb;
 }
 
+/* closest-hit stack entries the device keeps in LDS per lane */
+static uint32_t g_stack_lds = 16;
+void orc_set_stack_lds(uint32_t entries) { g_stack_lds = entries; }
+
 /* Two-level traversal. Returns 1 if an accepted hit was found. */
 static int traverse(qctx* q, v3 o, v3 d, float tmin, float tmax, int occlusion, hitrec* best) {
     uint32_t stack_ref[STACK_MAX];
@@ -268,10 +272,10 @@ static int traverse(qctx* q, v3 o, v3 d, float tmin, float tmax, int occlusion, 
     /* counters of this query kind: nodes, triangles, instances, any-hits */
     uint64_t* kc = occlusion ? &q->cnt.occ_node_visits : &q->cnt.node_visits;
     /* pushes at stack positions the device keeps in its per-lane spill area
-       (beyond 16 closest-hit or 32 occlusion entries in LDS); the push
-       order, hence every position, is the device's */
+       (beyond g_stack_lds closest-hit or twice as many occlusion entries in
+       LDS); the push order, hence every position, is the device's */
     uint64_t* ksp = occlusion ? &q->cnt.occ_spill_pushes : &q->cnt.spill_pushes;
-    const uint32_t lds_depth = occlusion ? 32u : 16u;
+    const uint32_t lds_depth = occlusion ? 2u * g_stack_lds : g_stack_lds;
     uint32_t cur = 0;
     int have = 1;
     while (1) {

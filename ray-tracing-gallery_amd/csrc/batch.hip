@@ -19,6 +19,7 @@
 #include <map>
 #include <vector>
 
+#include "rtg_device.hpp"
 #include "rtg_internal.hpp"
 
 namespace rtg {
@@ -33,8 +34,12 @@ static int check_tuning(const rtg_tuning& t) {
     RTG_REQUIRE(t.path_order <= 1, RTG_ERROR_INVALID_ARGUMENT, "tuning: path_order %u (0..1)", t.path_order);
     RTG_REQUIRE(t.bounce_blocks <= 4096, RTG_ERROR_INVALID_ARGUMENT, "tuning: bounce_blocks %u (0..4096)",
                 t.bounce_blocks);
-    RTG_REQUIRE(t.reserved[0] == 0 && t.reserved[1] == 0, RTG_ERROR_INVALID_ARGUMENT,
-                "tuning: reserved fields must be zero");
+    RTG_REQUIRE(t.stack_lds_entries == 0 || (t.stack_lds_entries >= (uint32_t)kStackLdsMin &&
+                                             t.stack_lds_entries <= (uint32_t)kStackLdsMax &&
+                                             t.stack_lds_entries % 4 == 0),
+                RTG_ERROR_INVALID_ARGUMENT, "tuning: stack_lds_entries %u (0, or 8..32 in steps of 4)",
+                t.stack_lds_entries);
+    RTG_REQUIRE(t.reserved == 0, RTG_ERROR_INVALID_ARGUMENT, "tuning: reserved field must be zero");
     return RTG_OK;
 }
 
@@ -48,6 +53,7 @@ static Tuning from_abi(const rtg_tuning& t, const Tuning& current) {
     u.refill_occlusion = (int)t.refill_occlusion;
     u.path_order = (int)t.path_order;
     u.bounce_blocks = (int)t.bounce_blocks;
+    u.stack_lds = t.stack_lds_entries ? (int)t.stack_lds_entries : kStackLdsDefault;
     return u;
 }
 
@@ -60,13 +66,16 @@ static rtg_tuning to_abi(const Tuning& u) {
     t.refill_occlusion = (uint32_t)u.refill_occlusion;
     t.path_order = (uint32_t)u.path_order;
     t.bounce_blocks = (uint32_t)u.bounce_blocks;
+    t.stack_lds_entries = (uint32_t)u.stack_lds;
     return t;
 }
 
 static bool operator<(const Tuning& a, const Tuning& b) {
-    int x[6] = {a.blocks_per_cu, a.feed_chunk, a.refill_closest, a.refill_occlusion, a.path_order, a.bounce_blocks};
-    int y[6] = {b.blocks_per_cu, b.feed_chunk, b.refill_closest, b.refill_occlusion, b.path_order, b.bounce_blocks};
-    return std::lexicographical_compare(x, x + 6, y, y + 6);
+    int x[7] = {a.blocks_per_cu, a.feed_chunk,    a.refill_closest, a.refill_occlusion,
+                a.path_order,    a.bounce_blocks, a.stack_lds};
+    int y[7] = {b.blocks_per_cu, b.feed_chunk,    b.refill_closest, b.refill_occlusion,
+                b.path_order,    b.bounce_blocks, b.stack_lds};
+    return std::lexicographical_compare(x, x + 7, y, y + 7);
 }
 
 // Owns a stream and an event pair for the autotuner's timed runs, which use
@@ -143,13 +152,14 @@ extern "C" int rtg_autotune(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t n,
     RTG_HIP(hipEventCreate(&rig.e1));
     // Coordinate search: each pass tries every value of one parameter with
     // the others at the best setting so far; two passes.
-    const std::vector<int> axes[6] = {
+    const std::vector<int> axes[7] = {
         {4, 5, 6, 8},         // blocks_per_cu
         {64, 128, 256},       // feed_chunk
         {64, 48, 32, 0},      // refill_closest
         {64, 48, 32, 0},      // refill_occlusion
         {1, 0},               // path_order
         {192, 128, 256, 64, 0},  // bounce_blocks
+        {16, 8, 12, 24, 32},     // stack_lds
     };
     std::map<Tuning, double> seen;
     Tuning cur = ctx->tune;
@@ -158,11 +168,11 @@ extern "C" int rtg_autotune(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t n,
     if (st != RTG_OK) return st;
     seen[cur] = cur_ms;
     for (int pass = 0; pass < 2; ++pass) {
-        for (int axis = 0; axis < 6; ++axis) {
+        for (int axis = 0; axis < 7; ++axis) {
             for (int value : axes[axis]) {
                 Tuning t = cur;
-                int* field[6] = {&t.blocks_per_cu, &t.feed_chunk, &t.refill_closest, &t.refill_occlusion,
-                                 &t.path_order, &t.bounce_blocks};
+                int* field[7] = {&t.blocks_per_cu, &t.feed_chunk, &t.refill_closest, &t.refill_occlusion,
+                                 &t.path_order,    &t.bounce_blocks, &t.stack_lds};
                 *field[axis] = value;
                 if (seen.count(t)) continue;
                 double ms = 0.0;

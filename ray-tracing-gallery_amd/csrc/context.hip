@@ -119,6 +119,8 @@ extern "C" int rtg_create(rtg_ctx** out, int device) {
     // RTG_FEED_CHUNK counts 64-ray batches taken per atomic
     ctx->tune.feed_chunk = 64 * env_int("RTG_FEED_CHUNK", ctx->tune.feed_chunk / 64, 1, 16);
     ctx->tune.bounce_blocks = env_int("RTG_BOUNCE_BLOCKS", ctx->tune.bounce_blocks, 0, 4096);
+    // RTG_STACK_LDS: stack entries per lane in LDS, rounded down to a multiple of 4
+    ctx->tune.stack_lds = env_int("RTG_STACK_LDS", ctx->tune.stack_lds, kStackLdsMin, kStackLdsMax) / 4 * 4;
     // IEC 61966-2-1 decode of the 256 sRGB codes, in double, rounded once.
     float lut[256];
     for (int i = 0; i < 256; ++i) {

@@ -521,31 +521,38 @@ __device__ inline bool alpha_clip_keep(const SceneView& S, const rtg_instance* s
 // distance (popped entries beyond the closest hit are skipped); an occlusion
 // entry is the 4-B reference alone, since an occlusion query never shortens
 // its interval (the first accepted hit ends it), so twice as many fit in the
-// same 32 KB of LDS per 256-lane workgroup. Both hold kStackMax entries.
-constexpr int kLdsDepth = 16;     // closest-hit entries in LDS
-constexpr int kLdsDepthOcc = 32;  // occlusion entries in LDS
-constexpr int kSpillDepth = 80;
-constexpr int kStackMax = kLdsDepth + kSpillDepth;
-// bytes of one lane's spill area
-constexpr size_t kSpillLaneBytes = (size_t)kSpillDepth * 8;
-// LDS bytes of a 256-lane workgroup's stacks (either kind)
-constexpr size_t kStackLdsBytes = (size_t)kLdsDepth * 8 * 256;
-static_assert(kLdsDepthOcc * 4 * 256 == kStackLdsBytes, "both stack kinds take the same LDS");
-static_assert((kStackMax - kLdsDepthOcc) * 4 <= (int)kSpillLaneBytes, "occlusion spill fits the lane's area");
+// same LDS. How many closest-hit entries a lane keeps in LDS is a launch
+// setting (8 to 32, 16 by default; a 256-lane workgroup takes entries x 8 B
+// x 256 of LDS); occlusion stacks keep twice as many. Both kinds hold
+// kStackMax entries in all.
+constexpr int kStackLdsMin = 8;
+constexpr int kStackLdsMax = 32;
+constexpr int kStackLdsDefault = 16;
+constexpr int kStackMax = 96;
+// bytes of one lane's spill area: the closest-hit entries past the fewest
+// kept in LDS
+constexpr size_t kSpillLaneBytes = (size_t)(kStackMax - kStackLdsMin) * 8;
+static_assert((kStackMax - 2 * kStackLdsMin) * 4 <= (int)kSpillLaneBytes, "occlusion spill fits the lane's area");
+// LDS bytes of a 256-lane workgroup's stacks with `entries` closest-hit
+// entries per lane (either kind)
+__host__ __device__ constexpr size_t stack_lds_bytes(int entries) { return (size_t)entries * 8 * 256; }
 
 template <bool OCC>
 struct Stack;
 
 template <>
 struct Stack<false> {
-    static constexpr int kLds = kLdsDepth;
     uint2* lds;      // lds[k * stride + lane]
     uint2* spill;    // spill[k - kLds], per lane
     int stride;
+    int kLds;        // entries in LDS
     int sp;
-    __device__ inline void init(void* lds_words, uint8_t* spill_area, uint32_t gtid, int lane, int lanes) {
+    // `entries`: the closest-hit entries per lane in LDS (the launch setting)
+    __device__ inline void init(void* lds_words, int entries, uint8_t* spill_area, uint32_t gtid, int lane,
+                                int lanes) {
         lds = static_cast<uint2*>(lds_words) + lane;
         stride = lanes;
+        kLds = entries;
         spill = reinterpret_cast<uint2*>(spill_area + (size_t)gtid * kSpillLaneBytes);
         sp = 0;
     }
@@ -567,14 +574,18 @@ struct Stack<false> {
 
 template <>
 struct Stack<true> {
-    static constexpr int kLds = kLdsDepthOcc;
     uint32_t* lds;
     uint32_t* spill;
     int stride;
+    int kLds;
     int sp;
-    __device__ inline void init(void* lds_words, uint8_t* spill_area, uint32_t gtid, int lane, int lanes) {
+    // `entries`: the closest-hit entries per lane in LDS; occlusion entries
+    // are half their size, so twice as many fit
+    __device__ inline void init(void* lds_words, int entries, uint8_t* spill_area, uint32_t gtid, int lane,
+                                int lanes) {
         lds = static_cast<uint32_t*>(lds_words) + lane;
         stride = lanes;
+        kLds = 2 * entries;
         spill = reinterpret_cast<uint32_t*>(spill_area + (size_t)gtid * kSpillLaneBytes);
         sp = 0;
     }
@@ -656,8 +667,8 @@ struct Traverser {
     }
 
     // pushes from stack position sp0 on that went past the LDS part
-    __device__ static inline uint32_t spilled(int sp0, int sp1) {
-        const int from = sp0 > StackT::kLds ? sp0 : StackT::kLds;
+    __device__ static inline uint32_t spilled(int sp0, int sp1, int lds) {
+        const int from = sp0 > lds ? sp0 : lds;
         return sp1 > from ? (uint32_t)(sp1 - from) : 0u;
     }
 
@@ -683,7 +694,7 @@ struct Traverser {
             return kRayActive;
         }
         if (st.sp + nh - 1 > kStackMax) return kRayOverflow;
-        if (COUNT) cnt.spills += spilled(st.sp, st.sp + nh - 1);
+        if (COUNT) cnt.spills += spilled(st.sp, st.sp + nh - 1, st.kLds);
         if (OCCLUSION) {
             // slot order: the lowest overlapped slot is visited next, the
             // others are pushed highest first
@@ -725,7 +736,7 @@ struct Traverser {
         }
         if (COUNT) cnt.insts++;
         if (st.sp >= kStackMax) return kRayOverflow;
-        if (COUNT) cnt.spills += spilled(st.sp, st.sp + 1);
+        if (COUNT) cnt.spills += spilled(st.sp, st.sp + 1, st.kLds);
         st.push(kSentinel, -__builtin_inff());
         float w2o[12];
         float4 m0 = gld_f4(rp), m1 = gld_f4(rp + 1), m2 = gld_f4(rp + 2);

@@ -112,6 +112,10 @@ struct Tuning {
     // closest-hit traversal, fell onto the critical path in some runs; 256
     // and more slow the shadow rays
     int bounce_blocks = 192;
+    // closest-hit stack entries per lane kept in LDS by every traversal
+    // launch (occlusion stacks keep twice as many 4-B entries); deeper
+    // entries spill to the lane's global area
+    int stack_lds = 16;
 };
 
 }  // namespace rtg

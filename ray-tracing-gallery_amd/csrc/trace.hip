@@ -101,6 +101,7 @@ struct Frame {
     int refill_closest;       // idle lanes that trigger a refill (0: rounds kernel)
     int refill_occlusion;
     uint32_t feed_chunk;      // queue entries a wave takes per atomic
+    int stack_lds;            // closest-hit stack entries per lane in LDS (the launches' LDS holds them)
     uint32_t side;            // 1 on the bounce stream (counts its shadow rays apart)
     // capacities of the path and shadow queues: sizes read from the device
     // are clamped to them and entries checked, so that a frame state driven
@@ -189,12 +190,12 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_rays(const uint8_t* tla
                                                               uint32_t sbt_offset, rtg_hit* hits, uint32_t* occ,
                                                               uint2* spill, uint32_t* feed_counter,
                                                               uint64_t* counters, uint32_t* status, int refill_min,
-                                                              uint32_t feed_chunk) {
-    __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
+                                                              uint32_t feed_chunk, int stack_lds) {
+    extern __shared__ uint2 s_stack[];
     const TlasView T = tlas_view(tlas);
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
     Stack<OCC> st;
-    st.init(s_stack, reinterpret_cast<uint8_t*>(spill), gtid, threadIdx.x, kTraceThreads);
+    st.init(s_stack, stack_lds, reinterpret_cast<uint8_t*>(spill), gtid, threadIdx.x, kTraceThreads);
     TravCounters cnt = {0, 0, 0, 0, 0};
     Traverser<OCC, COUNT> tr;
     WaveFeeder feeder;
@@ -314,7 +315,7 @@ __device__ __forceinline__ void trace_closest(Frame F, int level, const uint32_t
     const Queue Q = F.q[level & 1];
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
     Stack<false> st;
-    st.init(s_stack, reinterpret_cast<uint8_t*>(F.spill), gtid, threadIdx.x, kTraceThreads);
+    st.init(s_stack, F.stack_lds, reinterpret_cast<uint8_t*>(F.spill), gtid, threadIdx.x, kTraceThreads);
     TravCounters cnt = {0, 0, 0, 0, 0};
     Traverser<false, COUNT> tr;
     WaveFeeder feeder;
@@ -370,7 +371,7 @@ __device__ __forceinline__ void trace_closest(Frame F, int level, const uint32_t
 template <bool COUNT, bool SIDE>
 __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_trace_closest_one(
     Frame F, int level, const uint32_t* count_ptr, uint32_t fixed_count, uint32_t* status) {
-    __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
+    extern __shared__ uint2 s_stack[];
     trace_closest<COUNT, SIDE, true>(F, level, count_ptr, fixed_count, status, s_stack);
 }
 
@@ -379,7 +380,7 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5
 template <bool COUNT, bool SIDE>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int level, const uint32_t* count_ptr,
                                                                  uint32_t fixed_count, uint32_t* status) {
-    __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
+    extern __shared__ uint2 s_stack[];
     trace_closest<COUNT, SIDE, false>(F, level, count_ptr, fixed_count, status, s_stack);
 }
 
@@ -388,12 +389,12 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest(Frame F, int le
 template <bool COUNT, bool SIDE>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_closest_rounds(Frame F, int level, const uint32_t* count_ptr,
                                                                         uint32_t fixed_count, uint32_t* status) {
-    __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
+    extern __shared__ uint2 s_stack[];
     const uint32_t n = min(count_ptr ? *count_ptr : fixed_count, F.path_cap);
     const Queue Q = F.q[level & 1];
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
     Stack<false> st;
-    st.init(s_stack, reinterpret_cast<uint8_t*>(F.spill), gtid, threadIdx.x, kTraceThreads);
+    st.init(s_stack, F.stack_lds, reinterpret_cast<uint8_t*>(F.spill), gtid, threadIdx.x, kTraceThreads);
     TravCounters cnt = {0, 0, 0, 0, 0};
     const uint32_t stride = gridDim.x * kTraceThreads;
     const uint32_t rounds = (n + stride - 1) / stride;
@@ -726,14 +727,14 @@ __device__ inline void shadow_part(const Frame& F, int part, uint32_t& lo, uint3
 template <bool COUNT, bool SIDE>
 __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_trace_occlusion(
     Frame F, uint32_t* status, int part) {
-    __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
+    extern __shared__ uint2 s_stack[];
     uint32_t lo, hi;
     uint32_t* feed;
     shadow_part(F, part, lo, hi, feed);
     const uint32_t n = hi - lo;
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
     Stack<true> st;
-    st.init(s_stack, reinterpret_cast<uint8_t*>(F.spill), gtid, threadIdx.x, kTraceThreads);
+    st.init(s_stack, F.stack_lds, reinterpret_cast<uint8_t*>(F.spill), gtid, threadIdx.x, kTraceThreads);
     TravCounters cnt = {0, 0, 0, 0, 0};
     Traverser<true, COUNT> tr;
     WaveFeeder feeder;
@@ -785,14 +786,14 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5
 
 template <bool COUNT, bool SIDE>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion_rounds(Frame F, uint32_t* status, int part) {
-    __shared__ uint2 s_stack[kLdsDepth * kTraceThreads];
+    extern __shared__ uint2 s_stack[];
     uint32_t lo, hi;
     uint32_t* feed;
     shadow_part(F, part, lo, hi, feed);
     const uint32_t n = hi - lo;
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
     Stack<true> st;
-    st.init(s_stack, reinterpret_cast<uint8_t*>(F.spill), gtid, threadIdx.x, kTraceThreads);
+    st.init(s_stack, F.stack_lds, reinterpret_cast<uint8_t*>(F.spill), gtid, threadIdx.x, kTraceThreads);
     TravCounters cnt = {0, 0, 0, 0, 0};
     const uint32_t stride = gridDim.x * kTraceThreads;
     const uint32_t rounds = (n + stride - 1) / stride;
@@ -1187,11 +1188,13 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
     F.refill_closest = tune.refill_closest;
     F.refill_occlusion = tune.refill_occlusion;
     F.feed_chunk = (uint32_t)tune.feed_chunk;
+    F.stack_lds = tune.stack_lds;
     F.side = 0;
     F.path_cap = (uint32_t)std::min<uint64_t>(a->paths_cap, 0xffffffffu);
     F.shadow_cap = (uint32_t)std::min<uint64_t>(a->shadow_cap, 0xffffffffu);
     F.status = a->d_counts + kStatusSlot;
     const unsigned G = (unsigned)trace_grid(ctx, tune);
+    const size_t lds = stack_lds_bytes(tune.stack_lds);  // a traversal workgroup's stacks
     const unsigned Gsmall = (unsigned)std::min<uint64_t>((P.paths + 255) / 256, (uint64_t)ctx->cu_count * 16);
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, a->d_counts, ctx->d_counters, cnt);
     {
@@ -1210,16 +1213,16 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
             if (tune.refill_closest == 0) {
                 if (side)
                     hipLaunchKernelGGL((cnt ? k_trace_closest_rounds<true, true> : k_trace_closest_rounds<false, true>),
-                                       dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, Fl.status);
+                                       dim3(g), dim3(kTraceThreads), lds, st, Fl, (int)level, cp, fixed, Fl.status);
                 else
                     hipLaunchKernelGGL((cnt ? k_trace_closest_rounds<true, false> : k_trace_closest_rounds<false, false>),
-                                       dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, Fl.status);
+                                       dim3(g), dim3(kTraceThreads), lds, st, Fl, (int)level, cp, fixed, Fl.status);
             } else {
                 auto kernel = side ? (cnt ? (single ? k_trace_closest_one<true, true> : k_trace_closest<true, true>)
                                           : (single ? k_trace_closest_one<false, true> : k_trace_closest<false, true>))
                                    : (cnt ? (single ? k_trace_closest_one<true, false> : k_trace_closest<true, false>)
                                           : (single ? k_trace_closest_one<false, false> : k_trace_closest<false, false>));
-                hipLaunchKernelGGL(kernel, dim3(g), dim3(kTraceThreads), 0, st, Fl, (int)level, cp, fixed, Fl.status);
+                hipLaunchKernelGGL(kernel, dim3(g), dim3(kTraceThreads), lds, st, Fl, (int)level, cp, fixed, Fl.status);
             }
         }
         {
@@ -1238,16 +1241,16 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
         if (tune.refill_occlusion == 0) {
             if (side)
                 hipLaunchKernelGGL((cnt ? k_trace_occlusion_rounds<true, true> : k_trace_occlusion_rounds<false, true>),
-                                   dim3(g), dim3(kTraceThreads), 0, st, Fo, Fo.status, part);
+                                   dim3(g), dim3(kTraceThreads), lds, st, Fo, Fo.status, part);
             else
                 hipLaunchKernelGGL((cnt ? k_trace_occlusion_rounds<true, false> : k_trace_occlusion_rounds<false, false>),
-                                   dim3(g), dim3(kTraceThreads), 0, st, Fo, Fo.status, part);
+                                   dim3(g), dim3(kTraceThreads), lds, st, Fo, Fo.status, part);
         } else if (side) {
             hipLaunchKernelGGL((cnt ? k_trace_occlusion<true, true> : k_trace_occlusion<false, true>), dim3(g),
-                               dim3(kTraceThreads), 0, st, Fo, Fo.status, part);
+                               dim3(kTraceThreads), lds, st, Fo, Fo.status, part);
         } else {
             hipLaunchKernelGGL((cnt ? k_trace_occlusion<true, false> : k_trace_occlusion<false, false>), dim3(g),
-                               dim3(kTraceThreads), 0, st, Fo, Fo.status, part);
+                               dim3(kTraceThreads), lds, st, Fo, Fo.status, part);
         }
     };
     level_step(F, 0, G, Gsmall, s, timed);
@@ -1261,7 +1264,8 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
         // two grids use disjoint spill areas.
         const unsigned Gmain = G - Gside;
         Frame Fs = F;
-        Fs.spill = F.spill + (size_t)Gmain * kTraceThreads * kSpillDepth;
+        Fs.spill = reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(F.spill) +
+                                            (size_t)Gmain * kTraceThreads * kSpillLaneBytes);
         Fs.side = 1;
         hipLaunchKernelGGL(k_shadow_snapshot, dim3(1), dim3(64), 0, s, a->d_counts);
         hipEventRecord(a->fork, s);
@@ -1373,10 +1377,11 @@ static void launch_ray_query(rtg_ctx* ctx, rtg_tlas tlas, const rtg_scene_bindin
     if (cnt) hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, ctx->d_counters, (uint64_t)count);
     LaunchTimer t(ctx, mode == RTG_QUERY_CLOSEST ? RTG_KERNEL_TRACE_CLOSEST : RTG_KERNEL_TRACE_OCCLUSION, s, timed);
     cull_mask &= 0xffu;
-#define RTG_LAUNCH_RAYS(OCC, CNT)                                                                                   \
-    hipLaunchKernelGGL((k_trace_rays<OCC, CNT>), dim3(G), dim3(kTraceThreads), 0, s, tl, S, rays, count, ray_flags, \
-                       cull_mask, sbt_offset, hits, occluded, spill, feed, ctx->d_counters, ctx->d_status,      \
-                       ctx->tune.refill_rays, (uint32_t)ctx->tune.feed_chunk)
+#define RTG_LAUNCH_RAYS(OCC, CNT)                                                                                \
+    hipLaunchKernelGGL((k_trace_rays<OCC, CNT>), dim3(G), dim3(kTraceThreads), stack_lds_bytes(ctx->tune.stack_lds), \
+                       s, tl, S, rays, count, ray_flags, cull_mask, sbt_offset, hits, occluded, spill, feed,         \
+                       ctx->d_counters, ctx->d_status, ctx->tune.refill_rays, (uint32_t)ctx->tune.feed_chunk,       \
+                       ctx->tune.stack_lds)
     if (mode == RTG_QUERY_CLOSEST) {
         if (cnt) RTG_LAUNCH_RAYS(false, true);
         else RTG_LAUNCH_RAYS(false, false);

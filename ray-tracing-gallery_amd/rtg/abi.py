@@ -287,7 +287,8 @@ class Tuning(C.Structure):
         ("refill_occlusion", C.c_uint32),
         ("path_order", C.c_uint32),
         ("bounce_blocks", C.c_uint32),
-        ("reserved", C.c_uint32 * 2),
+        ("stack_lds_entries", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
     def as_dict(self):
@@ -295,8 +296,9 @@ class Tuning(C.Structure):
 
     @staticmethod
     def make(blocks_per_cu=4, feed_chunk=64, refill_closest=64, refill_occlusion=64, path_order=1,
-             bounce_blocks=128):
-        return Tuning(blocks_per_cu, feed_chunk, refill_closest, refill_occlusion, path_order, bounce_blocks)
+             bounce_blocks=128, stack_lds_entries=16):
+        return Tuning(blocks_per_cu, feed_chunk, refill_closest, refill_occlusion, path_order, bounce_blocks,
+                      stack_lds_entries)
 
 
 assert C.sizeof(Tuning) == 32

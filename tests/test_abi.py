@@ -84,13 +84,14 @@ def test_python_constants_equal_the_header_defines():
 
 
 def test_tuning_layout():
-    # rtg_tuning: six u32 settings + two reserved words
+    # rtg_tuning: seven u32 settings + one reserved word
     assert C.sizeof(abi.Tuning) == 32
     assert abi.Tuning.path_order.offset == 16
     assert abi.Tuning.bounce_blocks.offset == 20
+    assert abi.Tuning.stack_lds_entries.offset == 24
     t = abi.Tuning.make()
     assert t.as_dict() == {"blocks_per_cu": 4, "feed_chunk": 64, "refill_closest": 64, "refill_occlusion": 64,
-                           "path_order": 1, "bounce_blocks": 128}
+                           "path_order": 1, "bounce_blocks": 128, "stack_lds_entries": 16}
 
 
 MIRRORS = {

@@ -130,6 +130,9 @@ SETTINGS = [
     dict(bounce_blocks=0),
     dict(bounce_blocks=1),
     dict(blocks_per_cu=2, bounce_blocks=4096),
+    dict(stack_lds_entries=8),
+    dict(stack_lds_entries=32, blocks_per_cu=5),
+    dict(stack_lds_entries=12, refill_closest=0, refill_occlusion=0),
 ]
 
 
@@ -154,10 +157,17 @@ def test_tuning_never_changes_results(ctx, mini):
 def test_bad_tuning_is_refused(ctx):
     saved = ctx.tuning()
     for kw in [dict(blocks_per_cu=0), dict(blocks_per_cu=9), dict(feed_chunk=96), dict(refill_closest=65),
-               dict(path_order=2), dict(bounce_blocks=4097)]:
+               dict(path_order=2), dict(bounce_blocks=4097), dict(stack_lds_entries=4),
+               dict(stack_lds_entries=36), dict(stack_lds_entries=10)]:
         with pytest.raises(Exception):
             ctx.set_tuning(abi.Tuning.make(**kw))
     assert ctx.tuning().as_dict() == saved.as_dict()
+    # zero asks for the default depth
+    try:
+        ctx.set_tuning(abi.Tuning.make(**{**saved.as_dict(), "stack_lds_entries": 0}))
+        assert ctx.tuning().stack_lds_entries == 16
+    finally:
+        ctx.set_tuning(saved)
 
 
 def test_autotune(ctx, mini):
@@ -170,6 +180,7 @@ def test_autotune(ctx, mini):
         assert ms > 0
         assert ctx.tuning().as_dict() == best.as_dict()
         assert best.blocks_per_cu in (4, 5, 6, 8) and best.path_order in (0, 1)
+        assert best.stack_lds_entries in (8, 12, 16, 24, 32)
         _equal(before, _host(outs))
         _equal(before, _alone(ctx, dss))
     finally:

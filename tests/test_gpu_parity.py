@@ -136,6 +136,40 @@ def test_trace_rays_counters_match_oracle(ctx, small):
             assert getattr(c, f) == getattr(oc, f), f
 
 
+@pytest.mark.parametrize("entries", [8, 12, 24, 32])
+def test_stack_lds_entries_change_only_the_spill_counts(ctx, small, entries):
+    """With every LDS stack depth the queries give the oracle's hits, and
+    the pushes past the LDS part are the oracle's for that depth."""
+    s, ds, hs = small
+    rays = scenes.incoherent_rays(20000, (-10, -1, -10), (10, 4, 10), seed=13)
+    d = rays_to_device(rays)
+    saved = ctx.tuning()
+    t = abi.Tuning.make(**{**saved.as_dict(), "stack_lds_entries": entries})
+    ctx.set_tuning(t)
+    oh.lib().orc_set_stack_lds(entries)
+    try:
+        for mode, names in ((abi.RTG_QUERY_CLOSEST, ("node_visits", "spill_pushes")),
+                            (abi.RTG_QUERY_OCCLUSION, ("occ_node_visits", "occ_spill_pushes"))):
+            ctx.set_counters(True)
+            try:
+                out = ds.trace_rays(d, len(rays), mode)
+                c = ctx.counters()
+            finally:
+                ctx.set_counters(False)
+            ref, oc = hs.trace_rays(rays, mode=mode)
+            if mode == abi.RTG_QUERY_CLOSEST:
+                assert np.array_equal(hits_to_numpy(out).view(np.uint32), ref.view(np.uint32))
+            else:
+                assert np.array_equal(out.cpu().numpy().astype(np.uint32), ref)
+            for f in names:
+                assert getattr(c, f) == getattr(oc, f), f
+            if entries <= 12 and mode == abi.RTG_QUERY_CLOSEST:
+                assert c.spill_pushes > 0
+    finally:
+        oh.lib().orc_set_stack_lds(16)
+        ctx.set_tuning(saved)
+
+
 def _compare_image(ds, hs, frame=None, opts=None, rgb_tol=RGB_TOL):
     outs = ds.trace_image(frame_index=frame, opts=opts)
     torch.cuda.synchronize()

@@ -95,11 +95,11 @@ def test_results_alone_equal_results_in_the_mix(ctx, resident):
 
 
 def spill_area_bytes(ctx):
-    """Device bytes of one slot's traversal spill area: one 640-byte stack
+    """Device bytes of one slot's traversal spill area: one 704-byte stack
     spill per lane of the persistent grid (blocks_per_cu workgroups of 256
     lanes per CU)."""
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    return cus * ctx.tuning().blocks_per_cu * 256 * 640
+    return cus * ctx.tuning().blocks_per_cu * 256 * 704
 
 
 def test_budget_must_hold_every_slots_spill_area(ctx):
