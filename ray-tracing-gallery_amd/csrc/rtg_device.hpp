@@ -433,15 +433,13 @@ __device__ inline bool tri_test(const RayCtx& r, V3 p0, V3 p1, V3 p2, float tmin
     return true;
 }
 
-// Robust slab test of one slot (Ize 2013): near and far planes chosen by
-// the sign bit of the inverse direction (1/-0 = -inf counts as negative; a
-// NaN component makes both planes NaN); the far distance is inflated by
-// 1 + 2 gamma(3) before the comparison.
-__device__ inline bool slab(float lx, float hx, float ly, float hy, float lz, float hz, const RayCtx& r, bool sx,
-                            bool sy, bool sz, float tmin, float tcur, float& tn) {
-    float nx = sx ? hx : lx, fx = sx ? lx : hx;
-    float ny = sy ? hy : ly, fy = sy ? ly : hy;
-    float nz = sz ? hz : lz, fz = sz ? lz : hz;
+// Robust slab test of one slot (Ize 2013) against its near and far planes
+// per axis (the hi plane is the near one where the sign bit of the inverse
+// direction is set: 1/-0 = -inf counts as negative, and a NaN component
+// makes both distances NaN); the far distance is inflated by 1 + 2 gamma(3)
+// before the comparison.
+__device__ inline bool slab(float nx, float fx, float ny, float fy, float nz, float fz, const RayCtx& r, float tmin,
+                            float tcur, float& tn) {
     float tnx = (nx - r.o.x) * r.inv.x, tfx = (fx - r.o.x) * r.inv.x;
     float tny = (ny - r.o.y) * r.inv.y, tfy = (fy - r.o.y) * r.inv.y;
     float tnz = (nz - r.o.z) * r.inv.z, tfz = (fz - r.o.z) * r.inv.z;
@@ -677,17 +675,21 @@ struct Traverser {
     // others are pushed.
     __device__ inline int node_step(const TlasView& T, StackT& st, TravCounters& cnt) {
         const RayCtx& cr = in_blas ? obj : world;
+        // Each axis's lo and hi float4s lie side by side in the node, so the
+        // near and far planes of all four slots are picked by address (the
+        // sign bit of the inverse direction) instead of by a select per slot.
         const float4* np = reinterpret_cast<const float4*>(nodes + cur);
-        float4 LX = gld_f4(np), HX = gld_f4(np + 1), LY = gld_f4(np + 2), HY = gld_f4(np + 3);
-        float4 LZ = gld_f4(np + 4), HZ = gld_f4(np + 5);
+        const uint32_t ix = f2u(cr.inv.x) >> 31, iy = f2u(cr.inv.y) >> 31, iz = f2u(cr.inv.z) >> 31;
+        float4 NX = gld_f4(np + ix), FX = gld_f4(np + (ix ^ 1u));
+        float4 NY = gld_f4(np + 2 + iy), FY = gld_f4(np + 2 + (iy ^ 1u));
+        float4 NZ = gld_f4(np + 4 + iz), FZ = gld_f4(np + 4 + (iz ^ 1u));
         uint4 C = gld_u4(np + 6);
         if (COUNT) cnt.nodes++;
-        const bool sx = signbit(cr.inv.x), sy = signbit(cr.inv.y), sz = signbit(cr.inv.z);
         float t0, t1, t2, t3;
-        bool h0 = slab(LX.x, HX.x, LY.x, HY.x, LZ.x, HZ.x, cr, sx, sy, sz, tmin, tcur, t0) && C.x != kEmpty;
-        bool h1 = slab(LX.y, HX.y, LY.y, HY.y, LZ.y, HZ.y, cr, sx, sy, sz, tmin, tcur, t1) && C.y != kEmpty;
-        bool h2 = slab(LX.z, HX.z, LY.z, HY.z, LZ.z, HZ.z, cr, sx, sy, sz, tmin, tcur, t2) && C.z != kEmpty;
-        bool h3 = slab(LX.w, HX.w, LY.w, HY.w, LZ.w, HZ.w, cr, sx, sy, sz, tmin, tcur, t3) && C.w != kEmpty;
+        bool h0 = slab(NX.x, FX.x, NY.x, FY.x, NZ.x, FZ.x, cr, tmin, tcur, t0) && C.x != kEmpty;
+        bool h1 = slab(NX.y, FX.y, NY.y, FY.y, NZ.y, FZ.y, cr, tmin, tcur, t1) && C.y != kEmpty;
+        bool h2 = slab(NX.z, FX.z, NY.z, FY.z, NZ.z, FZ.z, cr, tmin, tcur, t2) && C.z != kEmpty;
+        bool h3 = slab(NX.w, FX.w, NY.w, FY.w, NZ.w, FZ.w, cr, tmin, tcur, t3) && C.w != kEmpty;
         const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
         if (nh == 0) {
             if (!pop(T, st)) return done();
