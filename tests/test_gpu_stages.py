@@ -104,3 +104,43 @@ def test_heatmap_mode_lies_on_the_ramp(ctx):
     torch.cuda.synchronize()
     assert not np.array_equal(out2[1].cpu().numpy(), plain[1].cpu().numpy())
     ds.destroy()
+
+
+def odd_texture_scene():
+    """The small scene with textures whose sizes are not powers of two: the
+    ground's base colour a 24x20 sRGB texture filtered bilinearly (its
+    UVs run to 4.3, and bilinear taps at the borders index -1), its
+    metallic-roughness a 17x13 UNORM texture sampled nearest, the blue-noise
+    mask 48x40, with two soft-shadow rays per hit."""
+    s = scenes.small_scene(width=160, height=120, n_tori=4)
+    rng = np.random.default_rng(0x0DD)
+    base = rng.integers(0, 256, (20, 24, 4), dtype=np.uint8)
+    mr = rng.integers(0, 256, (13, 17, 4), dtype=np.uint8)
+    # every 8-bit code appears in the UNORM textures' channels
+    mr.reshape(-1)[:256] = np.arange(256, dtype=np.uint8)
+    bn = rng.integers(0, 256, (40, 48, 4), dtype=np.uint8)
+    bn.reshape(-1)[:256] = np.arange(256, dtype=np.uint8)
+    d = s.add_texture(scenes.TextureData(24, 20, abi.RTG_FORMAT_RGBA8_SRGB, abi.RTG_FILTER_LINEAR, base, "odd base"))
+    m = s.add_texture(scenes.TextureData(17, 13, abi.RTG_FORMAT_RGBA8_UNORM, abi.RTG_FILTER_NEAREST, mr, "odd mr"))
+    s.textures[2] = scenes.TextureData(48, 40, abi.RTG_FORMAT_RGBA8_UNORM, abi.RTG_FILTER_NEAREST, bn, "odd noise")
+    g_mesh = s.add_mesh(scenes.quad(40.0, 40.0, 4.3, "odd ground"))
+    s.instances[0] = scenes.InstanceData(scenes.affine((0.0, 0.0, 0.0)), s.add_model(g_mesh, [(d, m, -1)]), g_mesh)
+    s.sun_radius = float(np.tan(np.radians(1.0)))
+    s.shadow_rays = 2
+    return s
+
+
+def test_textures_of_any_size_match_oracle(ctx):
+    """Texel wrapping and 8-bit decoding for texture sizes that are not
+    powers of two, with negative bilinear taps, against the oracle."""
+    s = odd_texture_scene()
+    ds = DeviceScene(ctx, s)
+    rgba, rgb, ph = ds.trace_image()
+    torch.cuda.synchronize()
+    o_rgba, o_rgb, o_hits, _ = oh.HostScene(s).trace_image()
+    assert np.array_equal(hits_to_numpy(ph).view(np.uint32), o_hits.view(np.uint32))
+    assert np.array_equal(rgba.cpu().numpy().view(np.uint32), o_rgba)
+    assert float(np.abs(rgb.cpu().numpy().reshape(-1, 3) - o_rgb).max()) <= RGB_TOL
+    # the ground fills much of the frame
+    assert (o_hits["instance"] == 0).mean() > 0.3
+    ds.destroy()
