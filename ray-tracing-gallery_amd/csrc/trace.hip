@@ -56,6 +56,18 @@ __device__ inline void order_to_pixel(uint32_t t, uint32_t W, uint32_t H, uint32
     }
 }
 
+// A job's path index local = t * spp + s: pixel-order index t and sample s.
+// Path indices are 32-bit (queue entries hold them), so 32-bit arithmetic.
+__device__ inline void split_sample(uint32_t local, uint32_t spp, uint32_t& t, uint32_t& s) {
+    if (spp == 1) {
+        t = local;
+        s = 0;
+        return;
+    }
+    t = local / spp;
+    s = local - t * spp;
+}
+
 struct JobDev {
     const uint8_t* tlas;
     SceneView scene;
@@ -272,10 +284,9 @@ __global__ __launch_bounds__(256) void k_raygen(Frame F, bool count) {
     for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < F.paths; p += stride) {
         uint32_t j = find_job(F.jobs, F.njobs, p, false);
         const JobDev& J = F.jobs[j];
-        uint64_t local = p - J.sample_base;
-        uint32_t s = (uint32_t)(local % J.spp);
-        uint32_t px, py;
-        order_to_pixel((uint32_t)(local / J.spp), J.width, J.height, px, py, F.order);
+        uint32_t t, s, px, py;
+        split_sample((uint32_t)(p - J.sample_base), J.spp, t, s);
+        order_to_pixel(t, J.width, J.height, px, py, F.order);
         float jx = 0.5f, jy = 0.5f;
         if (J.spp > 1) {
             jx = fractf(0.5f + (float)s * 0.754877669f);
@@ -454,14 +465,18 @@ __device__ inline HitGeom load_hit_geom(const SceneView& S, uint32_t custom, uin
 }
 
 // sample_directional_light (closest_hit_textured.glsl:77-94)
-__device__ inline V3 sample_dir_light(float rx, float ry, V3 c, float radius) {
+// The light's tangent frame (t, b) depends on its direction c alone, so a
+// hit computes it once for all its shadow rays.
+__device__ inline void light_frame(V3 c, V3& t, V3& b) {
+    t = normalize(cross(c, v3(0.0f, 1.0f, 0.0f)));
+    b = normalize(cross(t, c));
+}
+__device__ inline V3 sample_dir_light(float rx, float ry, V3 c, V3 t, V3 b, float radius) {
     float rad = sqrtf(rx);
     float angle = (ry * 2.0f) * kPi;
     float s, co;
     sincos_own(angle, &s, &co);
     float px = (rad * co) * radius, py = (rad * s) * radius;
-    V3 t = normalize(cross(c, v3(0.0f, 1.0f, 0.0f)));
-    V3 b = normalize(cross(t, c));
     return normalize((c + t * px) + b * py);
 }
 
@@ -527,9 +542,9 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
         }
         if (active) {
             J = &F.jobs[find_job(F.jobs, F.njobs, p, false)];
-            uint64_t local = p - J->sample_base;
-            uint32_t s = (uint32_t)(local % J->spp);
-            order_to_pixel((uint32_t)(local / J->spp), J->width, J->height, px, py, F.order);
+            uint32_t t_order, s;
+            split_sample((uint32_t)(p - J->sample_base), J->spp, t_order, s);
+            order_to_pixel(t_order, J->width, J->height, px, py, F.order);
             uint64_t out_index = ((uint64_t)py * J->width + px) * J->spp + s;
             frame = J->frame_index * J->spp + s;
             sun = v3(J->sun[0], J->sun[1], J->sun[2]);
@@ -665,6 +680,8 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
             // time before their stores, so that the loads overlap
             const rtg_texture bn = load_texture(S, bn_index);
             const float anim = (float)(frame % 32u) * 0.618033988749f;
+            V3 lt, lb;
+            light_frame(sun, lt, lb);
             for (uint32_t k0 = 0; k0 < want_shadow; k0 += 4) {
                 float n1[4], n2[4];
     #pragma unroll
@@ -684,7 +701,7 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
                     const uint32_t k = k0 + j;
                     if (k < want_shadow) {
                         float rx = fractf(n1[j] + anim), ry = fractf(n2[j] + anim);
-                        V3 sd = sample_dir_light(rx, ry, sun, sun_radius);
+                        V3 sd = sample_dir_light(rx, ry, sun, lt, lb, sun_radius);
                         uint32_t slot = sbase + k;
                         F.sq.o[slot] = make_float4(sh_origin.x, sh_origin.y, sh_origin.z, 0.001f);
                         F.sq.d[slot] = make_float4(sd.x, sd.y, sd.z, 10000.0f);
