@@ -815,12 +815,19 @@ struct Traverser {
     // the same iteration, so a wavefront's lanes, which need different kinds
     // of visits at once, each advance further per iteration. The visits a ray
     // makes and their order are those of one visit per iteration.
-    __device__ inline int step(const TlasView& T, const SceneView& S, StackT& st, TravCounters& cnt) {
+    // leaf_turn / inst_turn (wave-uniform) let a caller hold the leaf or the
+    // instance visits back for an iteration: a wave runs the code of a visit
+    // kind whenever one of its lanes needs it, and leaf and instance visits
+    // are rare per lane but needed by some lane in nearly every iteration.
+    __device__ inline int step(const TlasView& T, const SceneView& S, StackT& st, TravCounters& cnt,
+                               bool leaf_turn = true, bool inst_turn = true) {
         if ((cur & kLeafBit) && in_blas) {
+            if (!leaf_turn) return kRayActive;
             const int r = leaf_step(T, S, st, cnt);
             if (r != kRayActive) return r;
         }
         if ((cur & kLeafBit) && !in_blas) {
+            if (!inst_turn) return kRayActive;
             const int r = instance_step(T, st, cnt);
             if (r != kRayActive) return r;
         }

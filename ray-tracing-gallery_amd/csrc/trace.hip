@@ -214,7 +214,9 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_rays(const uint8_t* tla
     feeder.init(feed_chunk);
     bool active = false;
     uint32_t ray = 0;
+    uint32_t it = 0;  // iteration parity: leaf visits on even, instance entries on odd iterations
     while (true) {
+        ++it;
         uint64_t idle = __ballot(!active);
         if (want_refill(idle, feeder, refill_min)) {
             uint32_t id = feeder.feed(idle, !active, feed_counter, n);
@@ -228,7 +230,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_rays(const uint8_t* tla
         }
         if (__ballot(active) == 0) break;
         if (active) {
-            int res = tr.step(T, S, st, cnt);
+            int res = tr.step(T, S, st, cnt, (it & 1u) == 0u, (it & 1u) != 0u);
             if (res != kRayActive) {
                 if (res == kRayOverflow) atomicOr(status, (uint32_t)kStatusStackOverflow);
                 if (OCC) {
@@ -337,7 +339,9 @@ __device__ __forceinline__ void trace_closest(Frame F, int level, const uint32_t
     TlasView T;
     if (SINGLE) T = tlas_view(J->tlas);
     uint32_t c0 = 0;
+    uint32_t it = 0;  // iteration parity: leaf visits on even, instance entries on odd iterations
     while (true) {
+        ++it;
         uint64_t idle = __ballot(!active);
         if (want_refill(idle, feeder, F.refill_closest)) {
             uint32_t id = feeder.feed(idle, !active, &F.counts[kFeedLevel + level], n);
@@ -360,7 +364,7 @@ __device__ __forceinline__ void trace_closest(Frame F, int level, const uint32_t
         }
         if (__ballot(active) == 0) break;
         if (active) {
-            int res = tr.step(T, J->scene, st, cnt);
+            int res = tr.step(T, J->scene, st, cnt, (it & 1u) == 0u, (it & 1u) != 0u);
             if (res != kRayActive) {
                 if (res == kRayOverflow) atomicOr(status, (uint32_t)kStatusStackOverflow);
                 if (J->flags & RTG_TRACE_HEATMAP) {
@@ -761,7 +765,9 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5
     const JobDev* J = nullptr;
     TlasView T;
     uint32_t c0 = 0;
+    uint32_t it = 0;  // iteration parity: leaf visits on even, instance entries on odd iterations
     while (true) {
+        ++it;
         uint64_t idle = __ballot(!active);
         if (want_refill(idle, feeder, F.refill_occlusion)) {
             uint32_t id = feeder.feed(idle, !active, feed, n);
@@ -784,7 +790,7 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5
         }
         if (__ballot(active) == 0) break;
         if (active) {
-            int res = tr.step(T, J->scene, st, cnt);
+            int res = tr.step(T, J->scene, st, cnt, (it & 1u) == 0u, (it & 1u) != 0u);
             if (res != kRayActive) {
                 if (res == kRayOverflow) atomicOr(status, (uint32_t)kStatusStackOverflow);
                 if (J->flags & RTG_TRACE_HEATMAP) {
