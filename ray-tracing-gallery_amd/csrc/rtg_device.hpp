@@ -558,32 +558,32 @@ struct Stack;
 template <>
 struct Stack<false> {
     uint2* lds;      // lds[k * stride + lane]
-    uint2* spill;    // spill[k - kLds], per lane
+    uint2* spill;    // spill[k - lds_n], per lane
     int stride;
-    int kLds;        // entries in LDS
+    int lds_n;       // entries in LDS
     int sp;
     // `entries`: the closest-hit entries per lane in LDS (the launch setting)
     __device__ inline void init(void* lds_words, int entries, uint8_t* spill_area, uint32_t gtid, int lane,
                                 int lanes) {
         lds = static_cast<uint2*>(lds_words) + lane;
         stride = lanes;
-        kLds = entries;
+        lds_n = entries;
         spill = reinterpret_cast<uint2*>(spill_area + (size_t)gtid * kSpillLaneBytes);
         sp = 0;
     }
     __device__ inline void push(uint32_t ref, float tn) {
         uint2 e = make_uint2(ref, f2u(tn));
-        if (sp < kLds) {
+        if (sp < lds_n) {
             lds[sp * stride] = e;
         } else {
             rtg_u2v v = {e.x, e.y};
-            *(RTG_GLOBAL rtg_u2v*)(spill + (sp - kLds)) = v;
+            *(RTG_GLOBAL rtg_u2v*)(spill + (sp - lds_n)) = v;
         }
         ++sp;
     }
     __device__ inline uint2 pop() {
         --sp;
-        return sp < kLds ? lds[sp * stride] : gld_u2(spill + (sp - kLds));
+        return sp < lds_n ? lds[sp * stride] : gld_u2(spill + (sp - lds_n));
     }
 };
 
@@ -592,7 +592,7 @@ struct Stack<true> {
     uint32_t* lds;
     uint32_t* spill;
     int stride;
-    int kLds;
+    int lds_n;
     int sp;
     // `entries`: the closest-hit entries per lane in LDS; occlusion entries
     // are half their size, so twice as many fit
@@ -600,19 +600,19 @@ struct Stack<true> {
                                 int lanes) {
         lds = static_cast<uint32_t*>(lds_words) + lane;
         stride = lanes;
-        kLds = 2 * entries;
+        lds_n = 2 * entries;
         spill = reinterpret_cast<uint32_t*>(spill_area + (size_t)gtid * kSpillLaneBytes);
         sp = 0;
     }
     __device__ inline void push(uint32_t ref, float) {
-        if (sp < kLds) lds[sp * stride] = ref;
-        else *(RTG_GLOBAL uint32_t*)(spill + (sp - kLds)) = ref;
+        if (sp < lds_n) lds[sp * stride] = ref;
+        else *(RTG_GLOBAL uint32_t*)(spill + (sp - lds_n)) = ref;
         ++sp;
     }
     // (reference, -inf): an occlusion entry is never skipped
     __device__ inline uint2 pop() {
         --sp;
-        uint32_t r = sp < kLds ? lds[sp * stride] : gld_u32(spill + (sp - kLds));
+        uint32_t r = sp < lds_n ? lds[sp * stride] : gld_u32(spill + (sp - lds_n));
         return make_uint2(r, 0xff800000u);
     }
 };
@@ -713,7 +713,7 @@ struct Traverser {
             return kRayActive;
         }
         if (st.sp + nh - 1 > kStackMax) return kRayOverflow;
-        if (COUNT) cnt.spills += spilled(st.sp, st.sp + nh - 1, st.kLds);
+        if (COUNT) cnt.spills += spilled(st.sp, st.sp + nh - 1, st.lds_n);
         if (OCCLUSION) {
             // slot order: the lowest overlapped slot is visited next, the
             // others are pushed highest first
@@ -755,7 +755,7 @@ struct Traverser {
         }
         if (COUNT) cnt.insts++;
         if (st.sp >= kStackMax) return kRayOverflow;
-        if (COUNT) cnt.spills += spilled(st.sp, st.sp + 1, st.kLds);
+        if (COUNT) cnt.spills += spilled(st.sp, st.sp + 1, st.lds_n);
         st.push(kSentinel, -__builtin_inff());
         float w2o[12];
         float4 m0 = gld_f4(rp), m1 = gld_f4(rp + 1), m2 = gld_f4(rp + 2);
