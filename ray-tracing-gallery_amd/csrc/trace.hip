@@ -517,8 +517,10 @@ __device__ inline void brdf_parts(V3 n, V3 view, V3 l, float pr, V3 base, float 
     NoL_out = NoL;
 }
 
-// Hit and miss stages of trace level b.
-template <bool COUNT>
+// Hit and miss stages of trace level b. ONE: the frame state holds one job,
+// whose record (and TLAS header) every lane then reads as the same,
+// wave-uniform data instead of looking its job up per path.
+template <bool COUNT, bool ONE>
 __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_t* count_ptr, uint32_t fixed_count) {
     const uint32_t n = min(count_ptr ? *count_ptr : fixed_count, F.path_cap);
     const Queue Q = F.q[level & 1];
@@ -545,7 +547,7 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
             }
         }
         if (active) {
-            J = &F.jobs[find_job(F.jobs, F.njobs, p, false)];
+            J = ONE ? &F.jobs[0] : &F.jobs[find_job(F.jobs, F.njobs, p, false)];
             uint32_t t_order, s;
             split_sample((uint32_t)(p - J->sample_base), J->spp, t_order, s);
             order_to_pixel(t_order, J->width, J->height, px, py, F.order);
@@ -1250,10 +1252,10 @@ void enqueue_trace(rtg_ctx* ctx, FrameArena* a, const TracePlan& P, const Tuning
         }
         {
             LaunchTimer t(ctx, RTG_KERNEL_SHADE, st, tm);
-            if (cnt)
-                hipLaunchKernelGGL(k_shade<true>, dim3(gshade), dim3(256), 0, st, Fl, (int)level, cp, fixed);
-            else
-                hipLaunchKernelGGL(k_shade<false>, dim3(gshade), dim3(256), 0, st, Fl, (int)level, cp, fixed);
+            const bool one = Fl.njobs == 1;
+            auto kernel = cnt ? (one ? k_shade<true, true> : k_shade<true, false>)
+                              : (one ? k_shade<false, true> : k_shade<false, false>);
+            hipLaunchKernelGGL(kernel, dim3(gshade), dim3(256), 0, st, Fl, (int)level, cp, fixed);
         }
     };
     // occlusion rays of `part` of the shadow queue; SIDE kernels count into
