@@ -188,6 +188,41 @@ def _compare_image(ds, hs, frame=None, opts=None, rgb_tol=RGB_TOL):
     return o_rgb
 
 
+@pytest.mark.parametrize("entries", [8, 24])
+def test_frame_counters_with_other_stack_depths(ctx, entries):
+    """A frame of the reduced large scene (bounce levels, soft shadows,
+    alpha fences) traced with 8 or 24 LDS stack entries: the image and every
+    traversal counter, spill pushes included, equal the oracle's at that
+    depth."""
+    s = scenes.large_scene(width=160, height=90)
+    ds = DeviceScene(ctx, s)
+    hs = oh.HostScene(s)
+    saved = ctx.tuning()
+    ctx.set_tuning(abi.Tuning.make(**{**saved.as_dict(), "stack_lds_entries": entries}))
+    oh.lib().orc_set_stack_lds(entries)
+    try:
+        ctx.set_counters(True)
+        rgba, rgb, ph = ds.trace_image(frame_index=2)
+        gc = ctx.counters().as_dict()
+        ctx.set_counters(False)
+        torch.cuda.synchronize()
+        o_rgba, o_rgb, o_hits, oc = hs.trace_image(frame_index=2)
+    finally:
+        ctx.set_counters(False)
+        oh.lib().orc_set_stack_lds(16)
+        ctx.set_tuning(saved)
+    assert np.array_equal(hits_to_numpy(ph).view(np.uint32), o_hits.view(np.uint32))
+    assert np.array_equal(rgba.cpu().numpy().view(np.uint32), o_rgba)
+    assert float(np.abs(rgb.cpu().numpy().reshape(-1, 3) - o_rgb).max()) <= RGB_TOL
+    # the bounce stream keeps its own subset of the counters; the totals are
+    # what the oracle counts
+    for f in ("primary_rays", "bounce_rays", "shadow_rays", "node_visits", "triangle_tests", "instance_visits",
+              "anyhit_calls", "occ_node_visits", "occ_triangle_tests", "occ_instance_visits", "occ_anyhit_calls",
+              "spill_pushes", "occ_spill_pushes"):
+        assert gc[f] == getattr(oc, f), f
+    ds.destroy()
+
+
 def test_trace_image_small(ctx, small):
     s, ds, hs = small
     _compare_image(ds, hs)
