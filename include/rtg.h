@@ -55,7 +55,11 @@ typedef enum rtg_status {
     RTG_ERROR_STATE = -7,
     /* a served request's trace ran but set RTG_STATUS_STACK_OVERFLOW or
        RTG_STATUS_QUEUE_RANGE: its outputs are not valid (rtg_completion.status) */
-    RTG_ERROR_RESULT = -8
+    RTG_ERROR_RESULT = -8,
+    /* a served request was cancelled before its batch launched (rtg_completion.status) */
+    RTG_ERROR_CANCELLED = -9,
+    /* a served request's deadline passed before its batch launched (rtg_completion.status) */
+    RTG_ERROR_TIMEOUT = -10
 } rtg_status;
 
 /* ------------------------------------------------------------------------ */
@@ -537,13 +541,17 @@ typedef struct rtg_request {
     float* out_rgb;
     rtg_hit* out_primary_hits;
     uint64_t tag;                   /* returned in the completion */
+    double deadline_s;              /* server clock (rtg_server_clock) by which the request's batch must
+                                       launch, else it completes with RTG_ERROR_TIMEOUT without running;
+                                       0 = none */
 } rtg_request;
 
 typedef struct rtg_completion {
     uint64_t ticket;
     uint64_t tag;
-    int32_t status;    /* RTG_OK, RTG_ERROR_RESULT (the trace overflowed a traversal stack or dropped rays) or
-                          the launch error */
+    int32_t status;    /* RTG_OK, RTG_ERROR_RESULT (the trace overflowed a traversal stack or dropped rays),
+                          RTG_ERROR_CANCELLED, RTG_ERROR_TIMEOUT (neither ran: outputs untouched,
+                          batch_size 0, start_s = done_s) or the launch error */
     uint32_t batch_size;
     double submit_s;   /* server clock (seconds since rtg_server_create) */
     double start_s;    /* its batch was launched */
@@ -567,6 +575,11 @@ int rtg_server_add_scene(rtg_server* srv, rtg_tlas tlas, const rtg_scene_binding
 /* Queues a request (never blocks on the device). RTG_ERROR_BUDGET if it can
  * never fit one slot of the budget. */
 int rtg_server_submit(rtg_server* srv, const rtg_request* req, uint64_t* out_ticket);
+/* Cancels a request that is still pending (its batch not launched): it
+ * completes at once with RTG_ERROR_CANCELLED and its outputs are untouched.
+ * RTG_ERROR_NOT_FOUND if the ticket is not pending (launched, completed or
+ * unknown). */
+int rtg_server_cancel(rtg_server* srv, uint64_t ticket);
 /* Copies up to `capacity` completions; with wait_all, first waits until every
  * submitted request has completed. */
 int rtg_server_poll(rtg_server* srv, rtg_completion* out, uint32_t capacity, uint32_t* count, int wait_all);

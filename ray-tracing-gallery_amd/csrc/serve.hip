@@ -79,6 +79,19 @@ struct rtg_server {
     std::chrono::steady_clock::time_point t0;
 
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+    // completes a request that never ran (cancelled, timed out; mutex held)
+    void complete_unrun(const PendingRequest& p, int32_t status, double t) {
+        rtg_completion c{};
+        c.ticket = p.ticket;
+        c.tag = p.req.tag;
+        c.status = status;
+        c.batch_size = 0;
+        c.submit_s = p.submit_s;
+        c.start_s = t;
+        c.done_s = t;
+        completed.push_back(c);
+        ++finished;
+    }
     void run();
     bool launch(Slot& s);
     // device bytes held by all slots (frame buffers, spill areas, TLAS copies)
@@ -193,6 +206,17 @@ void rtg_server::run() {
             s.batch.clear();
             s.busy = false;
             progress = true;
+        }
+        // pending requests whose deadline has passed complete without running
+        const double t_now = now();
+        for (auto it = pending.begin(); it != pending.end();) {
+            if (it->req.deadline_s > 0.0 && t_now > it->req.deadline_s) {
+                complete_unrun(*it, RTG_ERROR_TIMEOUT, t_now);
+                it = pending.erase(it);
+                progress = true;
+            } else {
+                ++it;
+            }
         }
         if (progress) cv_done.notify_all();
         // admit pending requests into free slots
@@ -362,6 +386,22 @@ extern "C" int rtg_server_submit(rtg_server* srv, const rtg_request* req, uint64
     if (out_ticket) *out_ticket = p.ticket;
     srv->cv_work.notify_one();
     return RTG_OK;
+}
+
+extern "C" int rtg_server_cancel(rtg_server* srv, uint64_t ticket) {
+    clear_error();
+    RTG_REQUIRE(srv, RTG_ERROR_INVALID_ARGUMENT, "rtg_server_cancel: null server");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    for (auto it = srv->pending.begin(); it != srv->pending.end(); ++it) {
+        if (it->ticket != ticket) continue;
+        srv->complete_unrun(*it, RTG_ERROR_CANCELLED, srv->now());
+        srv->pending.erase(it);
+        srv->cv_done.notify_all();
+        return RTG_OK;
+    }
+    set_error("rtg_server_cancel: ticket %llu is not pending (launched, completed or unknown)",
+              (unsigned long long)ticket);
+    return RTG_ERROR_NOT_FOUND;
 }
 
 extern "C" int rtg_server_poll(rtg_server* srv, rtg_completion* out, uint32_t capacity, uint32_t* count,

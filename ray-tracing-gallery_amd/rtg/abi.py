@@ -324,6 +324,7 @@ class Request(C.Structure):
         ("out_rgb", C.c_uint64),
         ("out_primary_hits", C.c_uint64),
         ("tag", C.c_uint64),
+        ("deadline_s", C.c_double),
     ]
 
 
@@ -356,3 +357,5 @@ RTG_ERROR_NOT_FOUND = -5
 RTG_ERROR_BUDGET = -6
 RTG_ERROR_STATE = -7
 RTG_ERROR_RESULT = -8
+RTG_ERROR_CANCELLED = -9
+RTG_ERROR_TIMEOUT = -10

@@ -18,7 +18,8 @@ EXPORTS = [
     "rtg_tlas_query_sizes", "rtg_tlas_build", "rtg_tlas_refit", "rtg_tlas_clone", "rtg_tlas_destroy",
     "rtg_as_get_info", "rtg_trace_rays", "rtg_trace_rays_static", "rtg_trace_image", "rtg_trace_batch",
     "rtg_set_counters", "rtg_get_counters", "rtg_get_status", "rtg_set_timing", "rtg_get_kernel_time",
-    "rtg_server_create", "rtg_server_add_scene", "rtg_server_submit", "rtg_server_poll", "rtg_server_replay",
+    "rtg_server_create", "rtg_server_add_scene", "rtg_server_submit", "rtg_server_cancel", "rtg_server_poll",
+    "rtg_server_replay",
     "rtg_server_clock", "rtg_server_get_stats", "rtg_server_destroy",
     "rtg_get_tuning", "rtg_set_tuning", "rtg_autotune",
     "rtg_batch_create", "rtg_batch_update", "rtg_batch_launch", "rtg_batch_destroy",
@@ -67,6 +68,7 @@ def load():
     L.rtg_server_create.argtypes = [vp, C.POINTER(abi.ServerConfig), C.POINTER(vp)]
     L.rtg_server_add_scene.argtypes = [vp, u64, C.POINTER(abi.SceneBindings), C.POINTER(u32)]
     L.rtg_server_submit.argtypes = [vp, C.POINTER(abi.Request), C.POINTER(u64)]
+    L.rtg_server_cancel.argtypes = [vp, u64]
     L.rtg_server_poll.argtypes = [vp, C.POINTER(abi.Completion), u32, C.POINTER(u32), i32]
     L.rtg_server_replay.argtypes = [vp, C.POINTER(abi.Request), C.POINTER(C.c_double), u32,
                                     C.POINTER(abi.Completion)]

@@ -32,6 +32,14 @@ class Server:
         st = load().rtg_server_submit(self.ptr, C.byref(req), C.byref(t))
         return st, t.value
 
+    def cancel(self, ticket):
+        """RTG_OK if the request was still pending (it completes with
+        RTG_ERROR_CANCELLED), else RTG_ERROR_NOT_FOUND."""
+        return load().rtg_server_cancel(self.ptr, int(ticket))
+
+    def clock(self):
+        return float(load().rtg_server_clock(self.ptr))
+
     def poll(self, wait_all=False, capacity=4096):
         out = (abi.Completion * capacity)()
         n = C.c_uint32()
@@ -65,7 +73,7 @@ class Server:
 
 
 def make_request(scene_id, device_scene, width, height, spp, shadow_rays, max_traces, frame, instances=None,
-                 outputs=None, tag=0):
+                 outputs=None, tag=0, deadline_s=0.0):
     """A Request with the scene's camera and sun; `instances` is a device
     instance buffer (or None to trace the scene as built)."""
     r = abi.Request()
@@ -84,4 +92,5 @@ def make_request(scene_id, device_scene, width, height, spp, shadow_rays, max_tr
         r.out_rgb = rgb.data_ptr() if rgb is not None else 0
         r.out_primary_hits = hits.data_ptr() if hits is not None else 0
     r.tag = tag
+    r.deadline_s = float(deadline_s)
     return r

@@ -173,3 +173,45 @@ def test_sample_cap_bounds_batches(ctx, resident):
     for a, b in zip(snapshot(outs), capped):
         for x, y in zip(a, b):
             assert np.array_equal(x, y)
+
+
+def test_cancel_and_deadline(ctx, resident):
+    """A pending request can be cancelled, and one whose deadline passes
+    before its batch launches times out; neither runs (outputs untouched)
+    and both complete at once; the others complete as usual."""
+    srv = Server(ctx, budget_bytes=8 << 30, max_batch=1, max_inflight=1)
+    srv.add_scene(resident[0])
+
+    def outputs(w, h, spp):
+        return (torch.full((w * h,), -1, dtype=torch.int32, device="cuda:0"),
+                torch.full((w * h * 3,), -1.0, dtype=torch.float32, device="cuda:0"),
+                torch.full((w * h * spp * 6,), -1, dtype=torch.int32, device="cuda:0"))
+
+    # two large requests keep the single slot busy while the others queue
+    big = [make_request(0, resident[0], 1024, 1024, 4, 4, 3, frame=k, outputs=outputs(1024, 1024, 4), tag=k)
+           for k in range(2)]
+    ob, oc, od = outputs(40, 30, 1), outputs(40, 30, 1), outputs(40, 30, 1)
+    b = make_request(0, resident[0], 40, 30, 1, 2, 3, frame=5, outputs=ob, tag=10)
+    # a deadline already past (the server clock runs from its creation)
+    c = make_request(0, resident[0], 40, 30, 1, 2, 3, frame=6, outputs=oc, tag=11, deadline_s=1e-6)
+    d = make_request(0, resident[0], 40, 30, 1, 2, 3, frame=7, outputs=od, tag=12,
+                     deadline_s=srv.clock() + 600.0)
+    tickets = []
+    for r in big + [b, c, d]:
+        st, t = srv.submit(r)
+        assert st == 0
+        tickets.append(t)
+    assert srv.cancel(tickets[2]) == 0
+    assert srv.cancel(tickets[2]) == abi.RTG_ERROR_NOT_FOUND
+    assert srv.cancel(10 ** 9) == abi.RTG_ERROR_NOT_FOUND
+    done = {x.tag: x for x in srv.poll(wait_all=True)}
+    srv.close()
+    assert sorted(done) == [0, 1, 10, 11, 12]
+    assert done[0].status == 0 and done[1].status == 0 and done[12].status == 0
+    assert done[10].status == abi.RTG_ERROR_CANCELLED and done[10].batch_size == 0
+    assert done[11].status == abi.RTG_ERROR_TIMEOUT and done[11].batch_size == 0
+    assert done[10].start_s == done[10].done_s and done[11].start_s == done[11].done_s
+    torch.cuda.synchronize()
+    for o in (ob, oc):
+        assert bool((o[0] == -1).all()) and bool((o[1] == -1.0).all()) and bool((o[2] == -1).all())
+    assert not bool((od[0] == -1).all())
