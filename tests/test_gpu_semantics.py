@@ -3,7 +3,8 @@ test_oracle_trace.py, replayed through the HIP path with rtg_trace_rays:
 watertightness through vertices and edges, inclusive t range, the closest-hit
 tie-break, any-hit (alpha clip) under every combination of ray flags,
 instance flags and hit-group offsets, cull masks, axis-aligned and signed-zero
-directions, empty ranges and an empty TLAS. Hit records and occlusion must be
+directions, empty ranges, an empty TLAS and rays with non-finite or degenerate
+components. Hit records and occlusion must be
 bit-identical to the oracle's, and the expected outcome of each rule is checked
 as well, so that a rule both sides got wrong still fails."""
 import math
@@ -154,4 +155,47 @@ def test_empty_tlas_and_empty_query(ctx):
     out = p.ds.trace_rays(d, 0)
     torch.cuda.synchronize()
     assert out.numel() == 0
+    p.close()
+
+
+def test_non_finite_and_degenerate_rays(ctx):
+    """Rays with NaN or infinite components, a zero direction, tmin > tmax,
+    tmax = 0, a negative tmin or denormal direction components: each query
+    ends (no hang, no stack overflow) with the oracle's bits, and the oracle
+    answers them as the mathematics does (no hit from NaN rays or empty
+    ranges)."""
+    s = scenes.small_scene(width=8, height=8, n_tori=6)
+    p = Pair(ctx, s)
+    nan, inf, den = np.float32(np.nan), np.float32(np.inf), np.float32(1e-40)
+    cases = [
+        ((nan, 1.0, 0.0), (0.0, -1.0, 0.0), 0.0, 100.0),
+        ((0.0, 3.0, 0.0), (nan, -1.0, 0.0), 0.0, 100.0),
+        ((0.0, 3.0, 0.0), (0.0, -1.0, 0.0), nan, 100.0),
+        ((0.0, 3.0, 0.0), (0.0, -1.0, 0.0), 0.0, nan),
+        ((0.0, 3.0, 0.0), (0.0, 0.0, 0.0), 0.0, 100.0),
+        ((0.0, 3.0, 0.0), (0.0, -1.0, 0.0), 5.0, 1.0),
+        ((0.0, 3.0, 0.0), (0.0, -1.0, 0.0), 0.0, 0.0),
+        ((0.0, 3.0, 0.0), (0.0, -1.0, 0.0), -4.0, 100.0),
+        ((0.0, 3.0, 0.0), (den, -1.0, den), 0.0, inf),
+        ((inf, 3.0, 0.0), (-1.0, 0.0, 0.0), 0.0, inf),
+        ((0.0, 3.0, 0.0), (inf, -1.0, 0.0), 0.0, inf),
+        ((0.0, 1e30, 0.0), (0.0, -1.0, 0.0), 0.0, inf),
+        ((0.0, 3.0, 0.0), (0.0, -1e-30, 0.0), 0.0, inf),
+    ]
+    o = np.array([c[0] for c in cases], np.float32)
+    d = np.array([c[1] for c in cases], np.float32)
+    rays = np.zeros(len(cases), abi.RAY_DTYPE)
+    rays["origin"], rays["direction"] = o, d
+    rays["tmin"] = [c[2] for c in cases]
+    rays["tmax"] = [c[3] for c in cases]
+    ctx.status(clear=True)
+    h = p.trace(rays)
+    occ = p.trace(rays, abi.RTG_QUERY_OCCLUSION)
+    assert ctx.status(clear=True) == 0
+    # NaN components and empty ranges hit nothing
+    for k in (0, 1, 2, 3, 5, 6):
+        assert h["instance"][k] == abi.RTG_NO_HIT and occ[k] == 0, k
+    # the plain downward ray from y = 3 hits the ground from a negative tmin
+    # too, and with a tiny or denormal sideways drift
+    assert h["instance"][7] != abi.RTG_NO_HIT and h["instance"][8] != abi.RTG_NO_HIT
     p.close()
