@@ -42,8 +42,10 @@ def algorithmic_bytes(c, kind):
     move per the path's data layout: the ray (32 B) and its result (24 B hit /
     4 B flag), 128 B per 4-wide node record visited, 48 B per triangle record
     tested, 64 B instance record + 48 B world-to-object transform per BLAS
-    entered. The work of the bounce stream (the side_* counters) is left out:
-    it is timed apart."""
+    entered. Shadow rays are walked in pairs of one path's rays, and the
+    occlusion counters count a pair's record fetch once, so these are the
+    records the walks load. The work of the bounce stream (the side_*
+    counters) is left out: it is timed apart."""
     if kind == "closest":
         side = c["side_node_visits"] > 0
         rays = c["primary_rays"] + (0 if side else c["bounce_rays"])
@@ -668,6 +670,7 @@ def main():
               "roofline": roofline, "kernel_ms_per_frame": kernel_ms,
               "traversal": {"nodes_per_closest_ray": round(tot["node_visits"] / max(1, tot["primary_rays"] +
                                                                                     tot["bounce_rays"]), 2),
+                            # node records loaded per shadow ray (a pair walk's visit serves two rays)
                             "nodes_per_shadow_ray": round(tot["occ_node_visits"] / max(1, tot["shadow_rays"]), 2),
                             "spill_pushes_per_closest_ray": round(tot["spill_pushes"] / max(1, tot["primary_rays"] +
                                                                                             tot["bounce_rays"]), 4),

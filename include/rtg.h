@@ -369,7 +369,11 @@ typedef struct rtg_trace_counters {
     uint64_t triangle_tests;   /* triangle records tested */
     uint64_t instance_visits;  /* TLAS leaf -> BLAS transitions */
     uint64_t anyhit_calls;     /* alpha-clip evaluations */
-    /* occlusion traversals */
+    /* occlusion traversals: rtg_trace_rays counts per ray; an image's
+       shadow rays are walked in pairs of one path's rays (rays 2j and 2j+1,
+       an odd count's last alone), and a pair's node visit, triangle test or
+       instance entry counts once for both rays (one record fetch);
+       any-hit calls count per ray */
     uint64_t occ_node_visits;
     uint64_t occ_triangle_tests;
     uint64_t occ_instance_visits;
@@ -388,9 +392,11 @@ typedef struct rtg_trace_counters {
     uint64_t side_triangle_tests;
     uint64_t side_instance_visits;
     uint64_t side_anyhit_calls;
-    /* traversal-stack pushes that went past the part of the stack in LDS (16
-       closest-hit entries, 32 occlusion entries per lane) to the lane's spill
-       area in device memory; the side_* pair is the bounce stream's share */
+    /* traversal-stack pushes that went past the part of the stack in LDS
+       (rtg_tuning.stack_lds_entries 8-byte entries per lane, 16 by default;
+       twice as many 4-byte entries for rtg_trace_rays occlusion queries) to
+       the lane's spill area in device memory; the side_* pair is the bounce
+       stream's share */
     uint64_t spill_pushes;
     uint64_t occ_spill_pushes;
     uint64_t side_spill_pushes;
@@ -471,8 +477,8 @@ typedef struct rtg_tuning {
                                   0 = every level in sequence on the caller's stream */
     uint32_t stack_lds_entries; /* traversal stack entries per lane kept in LDS (8..32, multiple of 4;
                                   0 = 16): a workgroup takes entries * 2 KB of LDS, deeper entries
-                                  spill to global memory; occlusion stacks keep twice as many
-                                  (half-size) entries */
+                                  spill to global memory; the occlusion queries of rtg_trace_rays
+                                  keep twice as many (half-size) entries */
     uint32_t reserved;         /* zero */
 } rtg_tuning;
 int rtg_get_tuning(rtg_ctx* ctx, rtg_tuning* out);

@@ -54,10 +54,15 @@ int orc_tlas_refit(void* tlas_blob, const rtg_instance* instances, uint32_t coun
 void orc_free(void* blob);
 int orc_as_get_info(const void* blob, rtg_as_info* out);
 
-/* Closest-hit stack entries per lane the device keeps in LDS (its
-   rtg_tuning.stack_lds_entries; 16 by default; occlusion stacks keep twice
-   as many): sets which pushes the spill counters count. */
+/* Stack entries per lane the device keeps in LDS (its
+   rtg_tuning.stack_lds_entries; 16 by default; the single-ray occlusion
+   queries keep twice as many): sets which pushes the spill counters count. */
 void orc_set_stack_lds(uint32_t entries);
+/* 1 (the default): an image's shadow rays are walked in pairs of one path,
+   as the device traces them; 0: each alone with the single-ray occlusion
+   query. The visibility, hence the image, is the same either way; only the
+   occlusion counters differ. */
+void orc_set_shadow_pairs(int on);
 /* Ray queries (rtg_trace_rays semantics). threads <= 0 means all cores. */
 int orc_trace_rays(const void* tlas, const rtg_scene_bindings* scene, const rtg_ray* rays,
                    uint32_t count, uint32_t mode, uint32_t ray_flags, uint32_t cull_mask,

@@ -251,6 +251,9 @@ static int key_less(uint32_t ia, uint32_t ga, uint32_t pa, uint32_t ib, uint32_t
 /* closest-hit stack entries the device keeps in LDS per lane */
 static uint32_t g_stack_lds = 16;
 void orc_set_stack_lds(uint32_t entries) { g_stack_lds = entries; }
+/* shadow rays of an image in pairs (1) or one by one (0) */
+static int g_shadow_pairs = 1;
+void orc_set_shadow_pairs(int on) { g_shadow_pairs = on; }
 
 /* Two-level traversal. Returns 1 if an accepted hit was found. */
 static int traverse(qctx* q, v3 o, v3 d, float tmin, float tmax, int occlusion, hitrec* best) {
@@ -387,6 +390,136 @@ static int traverse(qctx* q, v3 o, v3 d, float tmin, float tmax, int occlusion, 
         if (!have) break;
     }
     return found;
+}
+
+/* Two shadow rays of one path (same origin and interval) walked together:
+ * the union of the nodes either ray overlaps, each visited once, stack
+ * entries carrying the mask of the rays that overlap them (8-byte entries on
+ * the device, so g_stack_lds of them fit in LDS). Children are taken in slot
+ * order (the lowest overlapped one next, the others pushed highest first), a
+ * ray leaves the mask once it has an accepted hit, and the walk ends when
+ * both have one or the stack is empty. Per ray the result is the single-ray
+ * occlusion query's (every node and triangle it overlaps is still tested,
+ * and a hit is accepted per ray); the counters count visits of the pair.
+ * m = 3 for a pair, 1 for ray 0 alone. Returns the mask of occluded rays,
+ * or -1 if the stack overflowed. The shadow rays' flags and hit-group
+ * offset are q's. */
+static int traverse_pair(qctx* q, v3 o, v3 d0, v3 d1, uint32_t m, float tmin, float tmax) {
+    uint32_t stack_ref[STACK_MAX];
+    uint32_t stack_mask[STACK_MAX];
+    uint32_t sp = 0;
+    oray world[2], obj[2];
+    oray_setup(&world[0], o, d0);
+    oray_setup(&world[1], o, d1);
+    const oray* cr = world;
+    const orc_node* nodes = q->tnodes;
+    const orc_tri* tris = NULL;
+    const orc_inst* cur_inst = NULL;
+    int in_blas = 0;
+    uint64_t* kc = &q->cnt.occ_node_visits;
+    uint32_t alive = m, mask = m, cur = 0;
+    int have = 1;
+    while (1) {
+        if (have) {
+            if (!(cur & ORC_LEAF_BIT)) {
+                const orc_node* nd = &nodes[cur];
+                kc[0]++;
+                uint32_t ref[4], msk[4];
+                int nh = 0;
+                for (int s = 0; s < 4; ++s) {
+                    if (nd->child[s] == ORC_EMPTY) continue;
+                    uint32_t c = 0;
+                    for (int r = 0; r < 2; ++r) {
+                        float t;
+                        if ((mask >> r) & 1u && box_test(nd, s, &cr[r], tmin, tmax, &t)) c |= 1u << r;
+                    }
+                    if (!c) continue;
+                    ref[nh] = nd->child[s];
+                    msk[nh] = c;
+                    ++nh;
+                }
+                if (nh == 0) {
+                    have = 0;
+                    continue;
+                }
+                for (int k = nh - 1; k >= 1; --k) {
+                    if (sp >= STACK_MAX) return -1;
+                    if (sp >= g_stack_lds) q->cnt.occ_spill_pushes++;
+                    stack_ref[sp] = ref[k];
+                    stack_mask[sp] = msk[k];
+                    ++sp;
+                }
+                cur = ref[0];
+                mask = msk[0];
+                continue;
+            }
+            if (!in_blas) {
+                const orc_inst* rec = &q->tinst[cur & 0x07ffffffu];
+                have = 0;
+                if (((rec->mask_flags & 0xffu) & q->cull_mask) == 0) continue;
+                const orc_header* bh = (const orc_header*)(uintptr_t)rec->blas;
+                if (!bh) continue;
+                kc[2]++;
+                if (sp >= STACK_MAX) return -1;
+                if (sp >= g_stack_lds) q->cnt.occ_spill_pushes++;
+                stack_ref[sp] = SENTINEL;
+                stack_mask[sp] = 0;
+                ++sp;
+                oray_setup(&obj[0], m34_point(rec->w2o, o), m34_dir(rec->w2o, d0));
+                oray_setup(&obj[1], m34_point(rec->w2o, o), m34_dir(rec->w2o, d1));
+                cr = obj;
+                nodes = (const orc_node*)((const uint8_t*)bh + bh->nodes_off);
+                tris = (const orc_tri*)((const uint8_t*)bh + bh->prims_off);
+                cur_inst = rec;
+                in_blas = 1;
+                cur = 0;
+                have = 1;
+                continue;
+            }
+            uint32_t first = cur & 0x07ffffffu;
+            uint32_t count = ((cur >> 27) & 0xfu) + 1u;
+            for (uint32_t k = first; k < first + count; ++k) {
+                const orc_tri* tr = &tris[k];
+                kc[1]++;
+                for (int r = 0; r < 2; ++r) {
+                    if (!((mask & alive) >> r & 1u)) continue;
+                    float t, u, v;
+                    if (!tri_test(&cr[r], tr->v0, tr->v1, tr->v2, tmin, tmax, &t, &u, &v)) continue;
+                    uint32_t geom = tr->geom & ~ORC_NONOPAQUE_BIT;
+                    uint32_t inst = cur_inst->index;
+                    uint32_t iflags = (cur_inst->mask_flags >> 8) & 0xffu;
+                    if (!is_opaque(tr->geom, iflags, q->ray_flags)) {
+                        uint32_t group = (q->tsrc[inst].sbt_offset_and_flags & 0xffffffu) + q->sbt_offset;
+                        if (group < RTG_HIT_GROUP_COUNT && !alpha_clip_keep(q, &kc[3], inst, geom, tr->prim, u, v))
+                            continue;
+                    }
+                    alive &= ~(1u << r);
+                }
+                if (alive == 0) return (int)m;
+            }
+            have = 0;
+            continue;
+        }
+        /* pop: a sentinel leaves the BLAS; entries no live ray overlaps are
+           dropped */
+        while (sp > 0) {
+            --sp;
+            uint32_t r = stack_ref[sp];
+            if (r == SENTINEL) {
+                in_blas = 0;
+                cr = world;
+                nodes = q->tnodes;
+                continue;
+            }
+            if ((stack_mask[sp] & alive) == 0) continue;
+            cur = r;
+            mask = stack_mask[sp] & alive;
+            have = 1;
+            break;
+        }
+        if (!have) break;
+    }
+    return (int)(m & ~alive);
 }
 
 static int qctx_init(qctx* q, const void* tlas, const rtg_scene_bindings* scene) {
@@ -959,14 +1092,30 @@ static v3 shade_textured(qctx* q, const rtg_image_job* job, uint32_t px, uint32_
     uint32_t saved_flags = q->ray_flags, saved_sbt = q->sbt_offset;
     q->ray_flags = RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RTG_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER;
     q->sbt_offset = 1;
-    for (uint32_t i = 0; i < nsh; ++i) {
+    /* the rays in pairs (2j, 2j + 1), an odd count's last alone, as the
+       device's occlusion lanes take them; a stack overflow leaves a pair's
+       rays unresolved, counted as not visible */
+    for (uint32_t i = 0; i < nsh; i += g_shadow_pairs ? 2 : 1) {
         float rx, ry;
         blue_noise_pair(bn, q->lut, px, py, i, frame, &rx, &ry);
-        v3 sd = sample_dir_light(rx, ry, sun, U->sun_radius);
-        hitrec sh;
-        q->cnt.shadow_rays++;
-        int occl = traverse(q, shadow_origin, sd, 0.001f, 10000.0f, 1, &sh);
-        vis = vis + (occl > 0 ? 0.0f : 1.0f);
+        v3 sd0 = sample_dir_light(rx, ry, sun, U->sun_radius), sd1 = sd0;
+        if (!g_shadow_pairs) {
+            hitrec sh;
+            q->cnt.shadow_rays++;
+            vis = vis + (traverse(q, shadow_origin, sd0, 0.001f, 10000.0f, 1, &sh) > 0 ? 0.0f : 1.0f);
+            continue;
+        }
+        uint32_t m = 1;
+        if (i + 1 < nsh) {
+            blue_noise_pair(bn, q->lut, px, py, i + 1, frame, &rx, &ry);
+            sd1 = sample_dir_light(rx, ry, sun, U->sun_radius);
+            m = 3;
+        }
+        q->cnt.shadow_rays += m == 3 ? 2 : 1;
+        int occl = traverse_pair(q, shadow_origin, sd0, sd1, m, 0.001f, 10000.0f);
+        if (occl < 0) occl = (int)m;
+        vis = vis + ((occl & 1) ? 0.0f : 1.0f);
+        if (m == 3) vis = vis + ((occl & 2) ? 0.0f : 1.0f);
     }
     q->ray_flags = saved_flags;
     q->sbt_offset = saved_sbt;

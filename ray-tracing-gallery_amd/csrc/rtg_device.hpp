@@ -536,10 +536,11 @@ __device__ __noinline__ bool alpha_clip_keep(const SceneView& S, const rtg_insta
 // distance (popped entries beyond the closest hit are skipped); an occlusion
 // entry is the 4-B reference alone, since an occlusion query never shortens
 // its interval (the first accepted hit ends it), so twice as many fit in the
-// same LDS. How many closest-hit entries a lane keeps in LDS is a launch
-// setting (8 to 32, 16 by default; a 256-lane workgroup takes entries x 8 B
-// x 256 of LDS); occlusion stacks keep twice as many. Both kinds hold
-// kStackMax entries in all.
+// same LDS. The pair walk of two shadow rays (PairTraverser) keeps 8-B
+// entries, a reference and the mask of the rays overlapping it. How many
+// 8-B entries a lane keeps in LDS is a launch setting (8 to 32, 16 by
+// default; a 256-lane workgroup takes entries x 8 B x 256 of LDS); 4-B
+// stacks keep twice as many. All kinds hold kStackMax entries in all.
 constexpr int kStackLdsMin = 8;
 constexpr int kStackLdsMax = 32;
 constexpr int kStackLdsDefault = 16;
@@ -834,6 +835,243 @@ struct Traverser {
         if (!(cur & kLeafBit)) return node_step(T, st, cnt);
         return kRayActive;
     }
+};
+
+// Two shadow rays of one path (one origin and interval, directions in the
+// light's cone) traced by one lane: they visit the union of the nodes either
+// ray overlaps, once, instead of each in a lane of its own. Stack entries
+// (8 B: reference, mask of the rays that overlap it) and visits follow the
+// single-ray occlusion order (depth first, children in slot order); a ray
+// leaves the pair's mask once it has an accepted hit, and the pair is done
+// when both have one or the stack is empty. Each ray's result is what the
+// single-ray query gives: the nodes and triangles tested for a ray are a
+// superset of those it overlaps, and a hit is accepted or not per ray.
+// A lane of mask 1 traces one ray alone (an odd shadow-ray count's last).
+template <bool COUNT>
+struct PairTraverser {
+    V3 wo, wd0, wd1, winv0, winv1;  // world rays (shared origin)
+    V3 oo, oinv0, oinv1;            // object-space rays in a BLAS
+    float sx0, sy0, sz0, sx1, sy1, sz1;
+    uint32_t k0, k1;                // shear axes kx | ky << 2 | kz << 4
+    const Node* nodes;
+    const TriRec* tris;
+    float tmin, tcur;
+    uint32_t inst_index, inst_flags;
+    uint32_t cur, mask, alive;
+    bool in_blas;
+    using StackT = Stack<false>;
+
+    // Shear of the watertight test (as ray_setup) for direction d.
+    __device__ static inline void shear(V3 d, V3 inv, float& sx, float& sy, float& sz, uint32_t& k) {
+        float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+        int kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+        int kx = kz + 1 == 3 ? 0 : kz + 1;
+        int ky = kx + 1 == 3 ? 0 : kx + 1;
+        if (sel3(d, kz) < 0.0f) {
+            int t = kx;
+            kx = ky;
+            ky = t;
+        }
+        sz = sel3(inv, kz);
+        sx = sel3(d, kx) * sz;
+        sy = sel3(d, ky) * sz;
+        k = (uint32_t)kx | ((uint32_t)ky << 2) | ((uint32_t)kz << 4);
+    }
+    __device__ static inline V3 inv3(V3 d) { return v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z); }
+
+    // m: 3 for a pair, 1 for ray 0 alone (d1 is then unused).
+    __device__ inline void start(const TlasView& T, V3 o, V3 d0, V3 d1, uint32_t m, float tmin_, float tmax_,
+                                 StackT& st) {
+        wo = o;
+        wd0 = d0;
+        wd1 = d1;
+        winv0 = inv3(d0);
+        winv1 = inv3(d1);
+        in_blas = false;
+        nodes = T.nodes;
+        tris = nullptr;
+        tmin = tmin_;
+        tcur = tmax_;
+        inst_index = inst_flags = 0;
+        st.sp = 0;
+        cur = 0;
+        mask = m;
+        alive = m;
+    }
+    // pushes from stack position sp0 on that went past the LDS part
+    __device__ static inline uint32_t spilled(int sp0, int sp1, int lds) {
+        const int from = sp0 > lds ? sp0 : lds;
+        return sp1 > from ? (uint32_t)(sp1 - from) : 0u;
+    }
+    __device__ inline bool pop(const TlasView& T, StackT& st) {
+        while (st.sp > 0) {
+            uint2 e = st.pop();
+            if (e.x == kSentinel) {
+                in_blas = false;
+                nodes = T.nodes;
+                continue;
+            }
+            if ((e.y & alive) == 0) continue;
+            cur = e.x;
+            mask = e.y & alive;
+            return true;
+        }
+        return false;
+    }
+    __device__ inline int node_step(const TlasView& T, StackT& st, TravCounters& cnt) {
+        const V3 o = in_blas ? oo : wo;
+        const V3 i0 = in_blas ? oinv0 : winv0;
+        const V3 i1 = in_blas ? oinv1 : winv1;
+        // slab planes picked by address for ray 0 (as in the single-ray
+        // visit); ray 1 uses them as they are when its direction has the
+        // same signs in every lane of the wave, else swaps them per axis
+        const float4* np = reinterpret_cast<const float4*>(nodes + cur);
+        const uint32_t ix = f2u(i0.x) >> 31, iy = f2u(i0.y) >> 31, iz = f2u(i0.z) >> 31;
+        float4 NX = gld_f4(np + ix), FX = gld_f4(np + (ix ^ 1u));
+        float4 NY = gld_f4(np + 2 + iy), FY = gld_f4(np + 2 + (iy ^ 1u));
+        float4 NZ = gld_f4(np + 4 + iz), FZ = gld_f4(np + 4 + (iz ^ 1u));
+        uint4 C = gld_u4(np + 6);
+        if (COUNT) cnt.nodes++;
+        RayCtx r0, r1;
+        r0.o = o;
+        r0.inv = i0;
+        r1.o = o;
+        r1.inv = i1;
+        float t;
+        const bool m0 = (mask & 1u) != 0, m1 = (mask & 2u) != 0;
+        const uint32_t jx = f2u(i1.x) >> 31, jy = f2u(i1.y) >> 31, jz = f2u(i1.z) >> 31;
+        uint32_t c0 = (m0 && slab(NX.x, FX.x, NY.x, FY.x, NZ.x, FZ.x, r0, tmin, tcur, t)) ? 1u : 0u;
+        uint32_t c1 = (m0 && slab(NX.y, FX.y, NY.y, FY.y, NZ.y, FZ.y, r0, tmin, tcur, t)) ? 1u : 0u;
+        uint32_t c2 = (m0 && slab(NX.z, FX.z, NY.z, FY.z, NZ.z, FZ.z, r0, tmin, tcur, t)) ? 1u : 0u;
+        uint32_t c3 = (m0 && slab(NX.w, FX.w, NY.w, FY.w, NZ.w, FZ.w, r0, tmin, tcur, t)) ? 1u : 0u;
+        if (__ballot(jx != ix || jy != iy || jz != iz) == 0) {
+            c0 |= (m1 && slab(NX.x, FX.x, NY.x, FY.x, NZ.x, FZ.x, r1, tmin, tcur, t)) ? 2u : 0u;
+            c1 |= (m1 && slab(NX.y, FX.y, NY.y, FY.y, NZ.y, FZ.y, r1, tmin, tcur, t)) ? 2u : 0u;
+            c2 |= (m1 && slab(NX.z, FX.z, NY.z, FY.z, NZ.z, FZ.z, r1, tmin, tcur, t)) ? 2u : 0u;
+            c3 |= (m1 && slab(NX.w, FX.w, NY.w, FY.w, NZ.w, FZ.w, r1, tmin, tcur, t)) ? 2u : 0u;
+        } else {
+            const bool sx = jx != ix, sy = jy != iy, sz = jz != iz;
+            const float4 AX = sx ? FX : NX, BX = sx ? NX : FX, AY = sy ? FY : NY, BY = sy ? NY : FY;
+            const float4 AZ = sz ? FZ : NZ, BZ = sz ? NZ : FZ;
+            c0 |= (m1 && slab(AX.x, BX.x, AY.x, BY.x, AZ.x, BZ.x, r1, tmin, tcur, t)) ? 2u : 0u;
+            c1 |= (m1 && slab(AX.y, BX.y, AY.y, BY.y, AZ.y, BZ.y, r1, tmin, tcur, t)) ? 2u : 0u;
+            c2 |= (m1 && slab(AX.z, BX.z, AY.z, BY.z, AZ.z, BZ.z, r1, tmin, tcur, t)) ? 2u : 0u;
+            c3 |= (m1 && slab(AX.w, BX.w, AY.w, BY.w, AZ.w, BZ.w, r1, tmin, tcur, t)) ? 2u : 0u;
+        }
+        if (C.x == kEmpty) c0 = 0u;
+        if (C.y == kEmpty) c1 = 0u;
+        if (C.z == kEmpty) c2 = 0u;
+        if (C.w == kEmpty) c3 = 0u;
+        const int nh = (int)(c0 != 0u) + (int)(c1 != 0u) + (int)(c2 != 0u) + (int)(c3 != 0u);
+        if (nh == 0) {
+            if (!pop(T, st)) return kRayMiss;
+            return kRayActive;
+        }
+        if (st.sp + nh - 1 > kStackMax) return kRayOverflow;
+        if (COUNT) cnt.spills += spilled(st.sp, st.sp + nh - 1, st.lds_n);
+        // the lowest overlapped slot is visited next, the others are
+        // pushed highest first, each with its mask
+        if (c3 && (c0 || c1 || c2)) st.push(C.w, u2f(c3));
+        if (c2 && (c0 || c1)) st.push(C.z, u2f(c2));
+        if (c1 && c0) st.push(C.y, u2f(c1));
+        cur = c0 ? C.x : c1 ? C.y : c2 ? C.z : C.w;
+        mask = c0 ? c0 : c1 ? c1 : c2 ? c2 : c3;
+        return kRayActive;
+    }
+    __device__ inline int instance_step(const TlasView& T, StackT& st, TravCounters& cnt) {
+        const InstRec* rec = T.recs + (cur & kFirstMask);
+        const float4* rp = reinterpret_cast<const float4*>(rec);
+        uint4 tail = gld_u4(rp + 3);  // blas lo, blas hi, index, mask_flags
+        uint32_t mf = tail.w;
+        const uint8_t* bb = reinterpret_cast<const uint8_t*>((uint64_t)tail.x | ((uint64_t)tail.y << 32));
+        if ((mf & 0xffu) == 0 || bb == nullptr) {  // shadow rays' cull mask is 0xff
+            if (!pop(T, st)) return kRayMiss;
+            return kRayActive;
+        }
+        if (COUNT) cnt.insts++;
+        if (st.sp >= kStackMax) return kRayOverflow;
+        if (COUNT) cnt.spills += spilled(st.sp, st.sp + 1, st.lds_n);
+        st.push(kSentinel, u2f(0u));
+        float w2o[12];
+        float4 m0 = gld_f4(rp), m1 = gld_f4(rp + 1), m2 = gld_f4(rp + 2);
+        w2o[0] = m0.x; w2o[1] = m0.y; w2o[2] = m0.z; w2o[3] = m0.w;
+        w2o[4] = m1.x; w2o[5] = m1.y; w2o[6] = m1.z; w2o[7] = m1.w;
+        w2o[8] = m2.x; w2o[9] = m2.y; w2o[10] = m2.z; w2o[11] = m2.w;
+        oo = m34_point(w2o, wo);
+        const V3 d0 = m34_dir(w2o, wd0), d1 = m34_dir(w2o, wd1);
+        oinv0 = inv3(d0);
+        oinv1 = inv3(d1);
+        shear(d0, oinv0, sx0, sy0, sz0, k0);
+        shear(d1, oinv1, sx1, sy1, sz1, k1);
+        const AsHeader* bh = reinterpret_cast<const AsHeader*>(bb);
+        nodes = reinterpret_cast<const Node*>(bb + gld_u64(&bh->nodes_off));
+        tris = reinterpret_cast<const TriRec*>(bb + gld_u64(&bh->prims_off));
+        inst_index = tail.z;
+        inst_flags = (mf >> 8) & 0xffu;
+        in_blas = true;
+        cur = 0;
+        return kRayActive;
+    }
+    __device__ inline int leaf_step(const TlasView& T, const SceneView& S, StackT& st, TravCounters& cnt) {
+        uint32_t first = cur & kFirstMask;
+        uint32_t count = ((cur >> 27) & 0xfu) + 1u;
+        for (uint32_t k = first; k < first + count; ++k) {
+            const float4* tp = reinterpret_cast<const float4*>(tris + k);
+            float4 a = gld_f4(tp), b = gld_f4(tp + 1), c = gld_f4(tp + 2);
+            if (COUNT) cnt.tris++;
+            for (uint32_t r = 0; r < 2; ++r) {
+                const uint32_t bit = 1u << r;
+                if (!(mask & alive & bit)) continue;
+                RayCtx rc;
+                rc.o = oo;
+                rc.sx = r ? sx1 : sx0;
+                rc.sy = r ? sy1 : sy0;
+                rc.sz = r ? sz1 : sz0;
+                const uint32_t kk = r ? k1 : k0;
+                rc.kx = (int)(kk & 3u);
+                rc.ky = (int)((kk >> 2) & 3u);
+                rc.kz = (int)((kk >> 4) & 3u);
+                float t, u, v;
+                if (!tri_test(rc, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), tmin, tcur, t, u, v))
+                    continue;
+                const uint32_t prim = f2u(a.w), gword = f2u(b.w);
+                const uint32_t geom = gword & ~kNonOpaqueBit;
+                if (!is_opaque(gword, inst_flags,
+                               RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RTG_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER)) {
+                    // hit group offset 1 (closest_hit_textured.glsl:159-172)
+                    uint32_t group = (gld_u32(&T.src[inst_index].sbt_offset_and_flags) & 0xffffffu) + 1u;
+                    if (group < RTG_HIT_GROUP_COUNT) {
+                        if (COUNT) cnt.anyhits++;
+                        if (!alpha_clip_keep(S, T.src, inst_index, geom, prim, u, v)) continue;
+                    }
+                }
+                alive &= ~bit;
+            }
+            if (alive == 0u) return kRayHit;
+        }
+        if (!pop(T, st)) return kRayMiss;
+        return kRayActive;
+    }
+    // One iteration, as Traverser::step. Returns kRayHit once both rays
+    // have a hit, kRayMiss once the stack is empty (occluded() then names
+    // the rays that have one), or kRayOverflow.
+    __device__ inline int step(const TlasView& T, const SceneView& S, StackT& st, TravCounters& cnt,
+                               bool leaf_turn = true, bool inst_turn = true) {
+        if ((cur & kLeafBit) && in_blas) {
+            if (!leaf_turn) return kRayActive;
+            const int r = leaf_step(T, S, st, cnt);
+            if (r != kRayActive) return r;
+        }
+        if ((cur & kLeafBit) && !in_blas) {
+            if (!inst_turn) return kRayActive;
+            const int r = instance_step(T, st, cnt);
+            if (r != kRayActive) return r;
+        }
+        if (!(cur & kLeafBit)) return node_step(T, st, cnt);
+        return kRayActive;
+    }
+    // bit r set: ray r has an accepted hit (given the mask it started with)
+    __device__ inline uint32_t occluded(uint32_t started) const { return started & ~alive; }
 };
 
 // Runs one ray to completion. Returns 1 for an accepted hit, 0 for none, -1

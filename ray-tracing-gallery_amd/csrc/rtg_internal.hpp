@@ -107,14 +107,14 @@ struct Tuning {
     int feed_chunk = 64;
     // persistent blocks that trace the bounce levels beside the first
     // level's shadow rays (0: all levels in sequence on the caller's stream);
-    // 192 of 1,024 measured best on the 1080p scene with two frames in
-    // flight: with 128 or 160 the bounce chain, slowed by the other frame's
-    // closest-hit traversal, fell onto the critical path in some runs; 256
-    // and more slow the shadow rays
-    int bounce_blocks = 192;
-    // closest-hit stack entries per lane kept in LDS by every traversal
-    // launch (occlusion stacks keep twice as many 4-B entries); deeper
-    // entries spill to the lane's global area
+    // 384 of 1,024 measured best on the 1080p scene with two frames in
+    // flight: with the shadow rays traced in pairs their launch is short
+    // enough that the bounce chain, slowed by the other frame's closest-hit
+    // traversal, sets the frame's pace with fewer blocks
+    int bounce_blocks = 384;
+    // stack entries per lane kept in LDS by every traversal launch (8-B
+    // entries; the single-ray occlusion queries of the ray API keep twice as
+    // many 4-B entries); deeper entries spill to the lane's global area
     int stack_lds = 16;
 };
 
@@ -169,6 +169,9 @@ void arena_reset(FrameArena* a);
 const uint32_t* arena_status_word(const FrameArena* a);
 void delete_frame_arena(FrameArena* a);
 size_t frame_bytes(uint64_t paths, uint64_t shadow, uint32_t njobs);
+// Shadow-queue slots a path with `shadow_rays` rays takes: an even number
+// (the occlusion kernel traces the slots in pairs of one path).
+inline uint64_t shadow_slots(uint32_t shadow_rays) { return (uint64_t)shadow_rays + (shadow_rays & 1u); }
 // The trace step of a batch of jobs on `stream` with the frame state in `a`
 // (the context's own arena if null); check_tlas validates each job's TLAS
 // against the context's structures.

@@ -108,7 +108,7 @@ namespace {
 // (the slot's spill area is charged apart, once per slot).
 uint64_t request_bytes(const rtg_request& r, const ServerScene& sc) {
     uint64_t paths = (uint64_t)r.width * r.height * r.opts.spp;
-    uint64_t bytes = frame_bytes(paths, paths * r.opts.shadow_rays, 1);
+    uint64_t bytes = frame_bytes(paths, paths * shadow_slots(r.opts.shadow_rays), 1);
     if (r.instances) bytes += (sc.tlas_bytes + 255) & ~uint64_t(255);
     return bytes;
 }
@@ -130,7 +130,7 @@ bool rtg_server::launch(Slot& s) {
         if (p.req.instances) copy_bytes += (p.scene.tlas_bytes + 255) & ~uint64_t(255);
         uint64_t rp = (uint64_t)p.req.width * p.req.height * p.req.opts.spp;
         paths += rp;
-        shadow += rp * p.req.opts.shadow_rays;
+        shadow += rp * shadow_slots(p.req.opts.shadow_rays);
     }
     // Buffers only grow, each to the largest batch in each dimension; when
     // that would take the slot past its share of the budget, the idle slot's
@@ -229,7 +229,7 @@ void rtg_server::run() {
                 const PendingRequest& p = pending.front();
                 const rtg_request& r = p.req;
                 uint64_t rp = (uint64_t)r.width * r.height * r.opts.spp;
-                uint64_t np = paths + rp, ns = shadow + rp * r.opts.shadow_rays;
+                uint64_t np = paths + rp, ns = shadow + rp * shadow_slots(r.opts.shadow_rays);
                 size_t nc = copies + (r.instances ? ((p.scene.tlas_bytes + 255) & ~uint64_t(255)) : 0);
                 uint64_t nb = frame_bytes(np, ns, (uint32_t)s.batch.size() + 1) + nc;
                 if (!s.batch.empty() && nb > slot_capacity) break;

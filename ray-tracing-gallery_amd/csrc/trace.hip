@@ -29,6 +29,8 @@ constexpr int kShadowFirst = 5;      // counts[5]: shadow rays of trace level 0
 constexpr int kFeedShadowRest = 13;  // counts[13]: feed counter of the shadow rays after them
 constexpr int kStatusSlot = 14;      // counts[14]: status bits of the trace (merged into the context's)
 constexpr int kPartAll = 0, kPartFirst = 1, kPartRest = 2;  // parts of the shadow queue
+// path of a shadow-queue padding slot (after an odd count of a path's rays)
+constexpr uint32_t kNoPath = 0xFFFFFFFFu;
 
 // Order of an image's pixels along its paths: bands of 8 rows, each band
 // walked column by column, so that the 64 lanes of a wavefront cover an 8x8
@@ -661,14 +663,17 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
                 want_next = nonzero && (uint32_t)(level + 1) < J->max_traces;
             }
         }
-        // compaction: one atomic per wave for each queue
-        uint32_t sbase = wave_alloc(want_shadow, &F.counts[4]);
+        // compaction: one atomic per wave for each queue; a path's shadow
+        // rays take an even number of slots (a padding slot after an odd
+        // count) so that the occlusion kernel's pairs never span two paths
+        const uint32_t shadow_slots = want_shadow + (want_shadow & 1u);
+        uint32_t sbase = wave_alloc(shadow_slots, &F.counts[4]);
         uint32_t nbase = wave_alloc(want_next ? 1u : 0u, next_count);
         if (want_next && nbase >= F.path_cap) {
             queue_range_error(F);
             want_next = false;
         }
-        if (want_shadow && sbase + want_shadow > F.shadow_cap) {
+        if (want_shadow && sbase + shadow_slots > F.shadow_cap) {
             queue_range_error(F);
             want_shadow = 0;
         }
@@ -715,6 +720,7 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
                     }
                 }
             }
+            if (want_shadow & 1u) F.sq.path[sbase + want_shadow] = kNoPath;
             n_shadow += want_shadow;
             n_textured++;
         }
@@ -731,9 +737,6 @@ __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_
     }
 }
 
-// Occlusion of the shadow rays (TerminateOnFirstHit | SkipClosestHit, hit
-// group offset 1, miss shader 1: closest_hit_textured.glsl:159-172);
-// shadow_ray_miss (lib.rs:33-36) makes the ray count as visible.
 // Parts of the shadow queue an occlusion launch traces: all of it, or the
 // rays of the first trace level (counts[kShadowFirst] of them, the queue's
 // size when the first shade finished) and the rays appended after them.
@@ -743,27 +746,57 @@ __device__ inline void shadow_part(const Frame& F, int part, uint32_t& lo, uint3
     feed = &F.counts[part == kPartRest ? kFeedShadowRest : kFeedShadow];
 }
 
-// At most 96 VGPRs (five waves per SIMD): beside four waves of this kernel a
-// SIMD then still holds one wave of the other frame in flight (closest-hit
-// traversal, 96 VGPRs, or shading, 116), which keeps the GPU fuller than
-// the four-wave allocation of 102 VGPRs did.
+// The pair of shadow-queue entries 2q, 2q + 1 of [lo, hi): a path's shadow
+// rays are allocated from an even slot, so a pair holds the rays 2j, 2j + 1
+// of one path, or ray 2j alone when the path has an odd count (its last slot
+// is padding, path kNoPath). Returns the pair's mask (3, or 1 for ray 2j
+// alone) and its rays.
+__device__ inline uint32_t shadow_pair(const Frame& F, uint32_t e0, uint32_t hi, uint32_t& path, float4& o,
+                                       float4& d0, float4& d1) {
+    path = F.sq.path[e0];
+    if (path >= F.paths) {
+        queue_range_error(F);
+        path = 0;
+    }
+    o = F.sq.o[e0];
+    d0 = F.sq.d[e0];
+    d1 = d0;
+    if (e0 + 1 >= hi) return 1u;
+    const uint32_t p1 = F.sq.path[e0 + 1];
+    if (p1 == kNoPath) return 1u;
+    if (p1 != path) {
+        // entries of different paths in one pair: a frame state driven by
+        // two traces at once; its second ray is dropped (reported visible)
+        queue_range_error(F);
+        F.sflag[e0 + 1] = 1;
+        return 1u;
+    }
+    d1 = F.sq.d[e0 + 1];
+    return 3u;
+}
+
+// Occlusion of the shadow rays (TerminateOnFirstHit | SkipClosestHit, hit
+// group offset 1, miss shader 1: closest_hit_textured.glsl:159-172);
+// shadow_ray_miss (lib.rs:33-36) makes the ray count as visible. Each lane
+// traces a pair of one path's shadow rays (PairTraverser); one byte per
+// shadow ray, plain stores (no atomics). A pair lane holds 121 VGPRs (four
+// waves per SIMD).
 template <bool COUNT, bool SIDE>
-__global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_trace_occlusion(
-    Frame F, uint32_t* status, int part) {
+__global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion(Frame F, uint32_t* status, int part) {
     extern __shared__ uint2 s_stack[];
     uint32_t lo, hi;
     uint32_t* feed;
     shadow_part(F, part, lo, hi, feed);
-    const uint32_t n = hi - lo;
+    const uint32_t npairs = (hi - lo + 1) / 2;
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
-    Stack<true> st;
+    Stack<false> st;
     st.init(s_stack, F.stack_lds, reinterpret_cast<uint8_t*>(F.spill), gtid, threadIdx.x, kTraceThreads);
     TravCounters cnt = {0, 0, 0, 0, 0};
-    Traverser<true, COUNT> tr;
+    PairTraverser<COUNT> tr;
     WaveFeeder feeder;
     feeder.init(F.feed_chunk);
     bool active = false;
-    uint32_t slot = 0;
+    uint32_t e0 = 0, started = 0, path = 0;
     const JobDev* J = nullptr;
     TlasView T;
     uint32_t c0 = 0;
@@ -772,20 +805,14 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5
         ++it;
         uint64_t idle = __ballot(!active);
         if (want_refill(idle, feeder, F.refill_occlusion)) {
-            uint32_t id = feeder.feed(idle, !active, feed, n);
-            if (id != ~0u) {
-                id += lo;
-                slot = id;
-                uint32_t path = F.sq.path[id];
-                if (path >= F.paths) {
-                    queue_range_error(F);
-                    path = 0;
-                }
+            uint32_t q = feeder.feed(idle, !active, feed, npairs);
+            if (q != ~0u) {
+                e0 = lo + 2 * q;
+                float4 o, d0, d1;
+                started = shadow_pair(F, e0, hi, path, o, d0, d1);
                 J = &F.jobs[find_job(F.jobs, F.njobs, path, false)];
                 T = tlas_view(J->tlas);
-                float4 a = F.sq.o[id], b = F.sq.d[id];
-                tr.start(T, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w,
-                         RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RTG_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER, 0xffu, 1u, st);
+                tr.start(T, v3(o.x, o.y, o.z), v3(d0.x, d0.y, d0.z), v3(d1.x, d1.y, d1.z), started, o.w, d0.w, st);
                 if (J->flags & RTG_TRACE_HEATMAP) c0 = (uint32_t)__builtin_readcyclecounter();
                 active = true;
             }
@@ -794,14 +821,13 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5
         if (active) {
             int res = tr.step(T, J->scene, st, cnt, (it & 1u) == 0u, (it & 1u) != 0u);
             if (res != kRayActive) {
+                // a stack overflow leaves both rays unresolved: not visible
+                const uint32_t occ = res == kRayOverflow ? started : tr.occluded(started);
                 if (res == kRayOverflow) atomicOr(status, (uint32_t)kStatusStackOverflow);
-                if (J->flags & RTG_TRACE_HEATMAP) {
-                    const uint32_t path = F.sq.path[slot];
-                    atomicAdd(&F.cycles[path < F.paths ? path : 0u], (uint32_t)__builtin_readcyclecounter() - c0);
-                }
-                // shadow_ray_miss (lib.rs:33-36): a miss counts as visible;
-                // one byte per shadow ray, plain stores (no atomics)
-                F.sflag[slot] = res == kRayMiss ? 1 : 0;
+                if (J->flags & RTG_TRACE_HEATMAP)
+                    atomicAdd(&F.cycles[path], (uint32_t)__builtin_readcyclecounter() - c0);
+                F.sflag[e0] = (occ & 1u) ? 0 : 1;
+                if (started & 2u) F.sflag[e0 + 1] = (occ & 2u) ? 0 : 1;
                 active = false;
             }
         }
@@ -809,36 +835,38 @@ __global__ __launch_bounds__(kTraceThreads) __attribute__((amdgpu_waves_per_eu(5
     if (COUNT) flush_counters(F.counters, cnt, SIDE ? kCountersOcclusionSide : kCountersOcclusion);
 }
 
+// The same pairs, one per lane and grid-stride round (refill_occlusion 0).
 template <bool COUNT, bool SIDE>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_occlusion_rounds(Frame F, uint32_t* status, int part) {
     extern __shared__ uint2 s_stack[];
     uint32_t lo, hi;
     uint32_t* feed;
     shadow_part(F, part, lo, hi, feed);
-    const uint32_t n = hi - lo;
+    const uint32_t npairs = (hi - lo + 1) / 2;
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
-    Stack<true> st;
+    Stack<false> st;
     st.init(s_stack, F.stack_lds, reinterpret_cast<uint8_t*>(F.spill), gtid, threadIdx.x, kTraceThreads);
     TravCounters cnt = {0, 0, 0, 0, 0};
     const uint32_t stride = gridDim.x * kTraceThreads;
-    const uint32_t rounds = (n + stride - 1) / stride;
+    const uint32_t rounds = (npairs + stride - 1) / stride;
     for (uint32_t r = 0; r < rounds; ++r) {
-        uint32_t i = lo + r * stride + gtid;
-        if (i < hi) {
-            uint32_t p = F.sq.path[i];
-            if (p >= F.paths) {
-                queue_range_error(F);
-                p = 0;
-            }
-            const JobDev& J = F.jobs[find_job(F.jobs, F.njobs, p, false)];
+        const uint32_t q = r * stride + gtid;
+        if (q < npairs) {
+            const uint32_t e0 = lo + 2 * q;
+            uint32_t path;
+            float4 o, d0, d1;
+            const uint32_t started = shadow_pair(F, e0, hi, path, o, d0, d1);
+            const JobDev& J = F.jobs[find_job(F.jobs, F.njobs, path, false)];
             const TlasView T = tlas_view(J.tlas);
-            float4 a = F.sq.o[i], b = F.sq.d[i];
-            HitRec h;
-            int res = traverse<true, COUNT>(T, J.scene, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w,
-                                            RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RTG_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER,
-                                            0xffu, 1u, h, st, cnt);
-            if (res < 0) atomicOr(status, (uint32_t)kStatusStackOverflow);
-            F.sflag[i] = res == 0 ? 1 : 0;
+            PairTraverser<COUNT> tr;
+            tr.start(T, v3(o.x, o.y, o.z), v3(d0.x, d0.y, d0.z), v3(d1.x, d1.y, d1.z), started, o.w, d0.w, st);
+            int res;
+            while ((res = tr.step(T, J.scene, st, cnt)) == kRayActive) {
+            }
+            const uint32_t occ = res == kRayOverflow ? started : tr.occluded(started);
+            if (res == kRayOverflow) atomicOr(status, (uint32_t)kStatusStackOverflow);
+            F.sflag[e0] = (occ & 1u) ? 0 : 1;
+            if (started & 2u) F.sflag[e0 + 1] = (occ & 2u) ? 0 : 1;
         }
     }
     if (COUNT) flush_counters(F.counters, cnt, SIDE ? kCountersOcclusionSide : kCountersOcclusion);
@@ -1144,7 +1172,7 @@ int plan_jobs(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t njobs, FrameAren
         uint64_t px = (uint64_t)jobs[k].width * jobs[k].height;
         P.paths += px * jobs[k].opts.spp;
         P.pixels += px;
-        P.shadow += px * jobs[k].opts.spp * jobs[k].opts.shadow_rays;
+        P.shadow += px * jobs[k].opts.spp * shadow_slots(jobs[k].opts.shadow_rays);
         P.max_traces = std::max(P.max_traces, jobs[k].opts.max_traces);
     }
     RTG_REQUIRE(P.paths < (1ull << 31) && P.shadow < (1ull << 32), RTG_ERROR_LIMIT,

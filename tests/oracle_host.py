@@ -36,6 +36,8 @@ def lib():
         L.orc_trace_rays_brute.argtypes = [vp, vp, vp, u32, u32, u32, u32, u32, vp, vp]
         L.orc_set_stack_lds.argtypes = [u32]
         L.orc_set_stack_lds.restype = None
+        L.orc_set_shadow_pairs.argtypes = [C.c_int]
+        L.orc_set_shadow_pairs.restype = None
         L.orc_trace_image.argtypes = [C.POINTER(abi.ImageJob), vp, u32, i32, C.POINTER(abi.TraceCounters)]
         L.orc_v_smith_ggx_correlated.argtypes = [C.c_float] * 3
         L.orc_v_smith_ggx_correlated.restype = C.c_float

@@ -188,6 +188,40 @@ def _compare_image(ds, hs, frame=None, opts=None, rgb_tol=RGB_TOL):
     return o_rgb
 
 
+@pytest.mark.parametrize("nsh,refill_occlusion", [(3, 64), (5, 0), (1, 64), (2, 0)])
+def test_shadow_ray_pairs_of_any_count(ctx, nsh, refill_occlusion):
+    """The occlusion kernels trace a path's shadow rays in pairs (an odd
+    count's last alone, after a padding slot in the queue), persistent
+    (refill 64) or one pair per lane and round (refill 0): the image and
+    every traversal counter equal the oracle's pair walk."""
+    s = scenes.large_scene(width=96, height=64, n_objects=120, n_fences=12, n_mirrors=6)
+    ds = DeviceScene(ctx, s)
+    hs = oh.HostScene(s)
+    o = s.trace_opts()
+    o.shadow_rays = nsh
+    o.spp = 2
+    saved = ctx.tuning()
+    ctx.set_tuning(abi.Tuning.make(**{**saved.as_dict(), "refill_occlusion": refill_occlusion}))
+    try:
+        ctx.set_counters(True)
+        rgba, rgb, ph = ds.trace_image(frame_index=4, opts=o)
+        gc = ctx.counters().as_dict()
+        ctx.set_counters(False)
+        torch.cuda.synchronize()
+        o_rgba, o_rgb, o_hits, oc = hs.trace_image(frame_index=4, opts=o)
+    finally:
+        ctx.set_counters(False)
+        ctx.set_tuning(saved)
+    assert np.array_equal(hits_to_numpy(ph).view(np.uint32), o_hits.view(np.uint32))
+    assert np.array_equal(rgba.cpu().numpy().view(np.uint32), o_rgba)
+    assert float(np.abs(rgb.cpu().numpy().reshape(-1, 3) - o_rgb).max()) <= RGB_TOL
+    assert oc.shadow_rays > 0 and oc.occ_node_visits > 0
+    for f in ("shadow_rays", "occ_node_visits", "occ_triangle_tests", "occ_instance_visits", "occ_anyhit_calls",
+              "occ_spill_pushes", "node_visits", "anyhit_calls"):
+        assert gc[f] == getattr(oc, f), f
+    ds.destroy()
+
+
 @pytest.mark.parametrize("entries", [8, 24])
 def test_frame_counters_with_other_stack_depths(ctx, entries):
     """A frame of the reduced large scene (bounce levels, soft shadows,
