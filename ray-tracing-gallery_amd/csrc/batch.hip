@@ -158,7 +158,7 @@ extern "C" int rtg_autotune(rtg_ctx* ctx, const rtg_image_job* jobs, uint32_t n,
         {64, 48, 32, 0},      // refill_closest
         {64, 48, 32, 0},      // refill_occlusion
         {1, 0},               // path_order
-        {384, 256, 512, 192, 0},  // bounce_blocks
+        {192, 384, 128, 256, 0},  // bounce_blocks
         {16, 8, 12, 24, 32},     // stack_lds
     };
     std::map<Tuning, double> seen;

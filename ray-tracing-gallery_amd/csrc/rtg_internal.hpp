@@ -105,13 +105,15 @@ struct Tuning {
     int path_order = 1;
     int blocks_per_cu = 4;
     int feed_chunk = 64;
-    // persistent blocks that trace the bounce levels beside the first
-    // level's shadow rays (0: all levels in sequence on the caller's stream);
-    // 384 of 1,024 measured best on the 1080p scene with two frames in
-    // flight: with the shadow rays traced in pairs their launch is short
-    // enough that the bounce chain, slowed by the other frame's closest-hit
-    // traversal, sets the frame's pace with fewer blocks
-    int bounce_blocks = 384;
+    // persistent blocks given to the bounce stream beside the first level's
+    // shadow rays (0: all levels in sequence on the caller's stream): its
+    // shadow-ray launch takes them all, its bounce-level launches at most
+    // 128, and the first level's shadow rays the rest of the grid. 192 of
+    // 1,024 measured best on the 1080p scene with two frames in flight
+    // (3,668-3,738 Mrays/s in seven runs on two boxes); 224, 256 and 320
+    // often settled into a slower overlap of the two frames, 384 once in
+    // seven runs, and 448 or more slowed the first level's shadow rays
+    int bounce_blocks = 192;
     // stack entries per lane kept in LDS by every traversal launch (8-B
     // entries; the single-ray occlusion queries of the ray API keep twice as
     // many 4-B entries); deeper entries spill to the lane's global area

@@ -296,7 +296,7 @@ class Tuning(C.Structure):
 
     @staticmethod
     def make(blocks_per_cu=4, feed_chunk=64, refill_closest=64, refill_occlusion=64, path_order=1,
-             bounce_blocks=384, stack_lds_entries=16):
+             bounce_blocks=192, stack_lds_entries=16):
         return Tuning(blocks_per_cu, feed_chunk, refill_closest, refill_occlusion, path_order, bounce_blocks,
                       stack_lds_entries)
 

@@ -91,7 +91,7 @@ def test_tuning_layout():
     assert abi.Tuning.stack_lds_entries.offset == 24
     t = abi.Tuning.make()
     assert t.as_dict() == {"blocks_per_cu": 4, "feed_chunk": 64, "refill_closest": 64, "refill_occlusion": 64,
-                           "path_order": 1, "bounce_blocks": 384, "stack_lds_entries": 16}
+                           "path_order": 1, "bounce_blocks": 192, "stack_lds_entries": 16}
 
 
 MIRRORS = {
