@@ -471,9 +471,10 @@ typedef struct rtg_tuning {
                                   0 = non-persistent kernel (grid-stride rounds) */
     uint32_t refill_occlusion; /* the same for occlusion rays */
     uint32_t path_order;       /* 0 = row-major paths, 1 = 8x8 pixel blocks per wavefront */
-    uint32_t bounce_blocks;    /* persistent workgroups (0..4096) that trace the bounce levels, and the
-                                  shadow rays they add, on a second stream beside the first level's
-                                  shadow rays (taken from the blocks_per_cu grid, at most half of it);
+    uint32_t bounce_blocks;    /* persistent workgroups (0..4096) of a second stream that traces the
+                                  bounce levels (on at most 128 of them) and the shadow rays they add
+                                  (on all) beside the first level's shadow rays (taken from the
+                                  blocks_per_cu grid, at most half of it); 192 by default;
                                   0 = every level in sequence on the caller's stream */
     uint32_t stack_lds_entries; /* traversal stack entries per lane kept in LDS (8..32, multiple of 4;
                                   0 = 16): a workgroup takes entries * 2 KB of LDS, deeper entries
