@@ -185,3 +185,48 @@ def test_autotune(ctx, mini):
         _equal(before, _alone(ctx, dss))
     finally:
         ctx.set_tuning(saved)
+
+
+def test_jobs_with_different_shadow_ray_counts(ctx, mini):
+    """One batch whose jobs cast 1, 3, 2 and 5 shadow rays per textured hit
+    (odd counts leave a padding slot after a path's rays, and the occlusion
+    kernel looks each pair's job up): every job equals itself traced alone
+    and the oracle, and the batch's shadow-ray count is the jobs' sum."""
+    ws, dss = mini
+    counts = [1, 3, 2, 5]
+    sel = list(range(len(counts)))
+    opts = []
+    for k, n in zip(sel, counts):
+        o = ws[k].trace_opts()
+        o.shadow_rays = n
+        opts.append(o)
+
+    def jobs_of():
+        outs, jobs = [], []
+        for k, o in zip(sel, opts):
+            out = dss[k].make_outputs(spp=o.spp)
+            outs.append(out)
+            jobs.append(dss[k].job(out, frame_index=3, opts=o))
+        return outs, jobs
+
+    outs, jobs = jobs_of()
+    for j in jobs:
+        check(load().rtg_trace_image(ctx.ptr, C.byref(j), None), "rtg_trace_image")
+    alone = _host(outs)
+    outs, jobs = jobs_of()
+    ctx.set_counters(True)
+    try:
+        trace_batch(ctx, jobs)
+        c = ctx.counters()
+    finally:
+        ctx.set_counters(False)
+    batched = _host(outs)
+    _equal(alone, batched)
+    shadow = 0
+    for k, o, got in zip(sel, opts, batched):
+        hs = oh.HostScene(ws[k])
+        rgba, rgb, hits, oc = hs.trace_image(frame_index=3, opts=o)
+        assert np.array_equal(got[0].view(np.uint32), rgba)
+        assert np.array_equal(got[2].view(np.uint32).reshape(-1), hits.view(np.uint32).reshape(-1))
+        shadow += oc.shadow_rays
+    assert c.shadow_rays == shadow > 0
