@@ -63,6 +63,10 @@ void orc_set_stack_lds(uint32_t entries);
    query. The visibility, hence the image, is the same either way; only the
    occlusion counters differ. */
 void orc_set_shadow_pairs(int on);
+/* The visits of one shadow ray's walk, or of a pair walk (test hook; see
+   orc_trace.c). */
+int orc_debug_visits(const void* tlas, const rtg_scene_bindings* scene, const float o[3], const float d0[3],
+                     const float d1[3], int pair, uint64_t* log, uint32_t cap, uint32_t* n);
 /* Ray queries (rtg_trace_rays semantics). threads <= 0 means all cores. */
 int orc_trace_rays(const void* tlas, const rtg_scene_bindings* scene, const rtg_ray* rays,
                    uint32_t count, uint32_t mode, uint32_t ray_flags, uint32_t cull_mask,

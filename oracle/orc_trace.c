@@ -39,7 +39,19 @@ typedef struct {
     float lut[256];
     uint32_t ray_flags, cull_mask, sbt_offset;
     rtg_trace_counters cnt;
+    /* test hook (orc_debug_visits): when set, every node visit, triangle
+       test and instance entry appends (kind, instance record or 0, record) */
+    uint64_t* vlog;
+    uint32_t vlog_n, vlog_cap;
 } qctx;
+
+static void visit_log(qctx* q, uint32_t kind, const void* inst, const void* rec) {
+    if (!q->vlog || q->vlog_n >= q->vlog_cap) return;
+    uint64_t* e = q->vlog + 3u * q->vlog_n++;
+    e[0] = kind;
+    e[1] = (uint64_t)(uintptr_t)inst;
+    e[2] = (uint64_t)(uintptr_t)rec;
+}
 
 /* ---------------------------------------------------------------- */
 /* Textures: the sampler2D lookups of the hit shaders (LOD 0, REPEAT). */
@@ -286,6 +298,7 @@ static int traverse(qctx* q, v3 o, v3 d, float tmin, float tmax, int occlusion, 
             if (!(cur & ORC_LEAF_BIT)) {
                 const orc_node* nd = &nodes[cur];
                 kc[0]++;
+                visit_log(q, 0, in_blas ? cur_inst : NULL, nd);
                 /* the overlapped slots in ascending (entry distance, slot)
                    order for closest-hit queries, in slot order for
                    occlusion queries (any hit ends them, so no order is
@@ -327,6 +340,7 @@ static int traverse(qctx* q, v3 o, v3 d, float tmin, float tmax, int occlusion, 
                 const orc_header* bh = (const orc_header*)(uintptr_t)rec->blas;
                 if (!bh) continue;
                 kc[2]++;
+                visit_log(q, 2, NULL, rec);
                 if (sp >= STACK_MAX) return -1;
                 if (sp >= lds_depth) ++*ksp;
                 stack_ref[sp] = SENTINEL;
@@ -347,6 +361,7 @@ static int traverse(qctx* q, v3 o, v3 d, float tmin, float tmax, int occlusion, 
             for (uint32_t k = first; k < first + count; ++k) {
                 const orc_tri* tr = &tris[k];
                 kc[1]++;
+                visit_log(q, 1, cur_inst, tr);
                 float t, u, v;
                 if (!tri_test(cr, tr->v0, tr->v1, tr->v2, tmin, tcur, &t, &u, &v)) continue;
                 uint32_t geom = tr->geom & ~ORC_NONOPAQUE_BIT;
@@ -424,6 +439,7 @@ static int traverse_pair(qctx* q, v3 o, v3 d0, v3 d1, uint32_t m, float tmin, fl
             if (!(cur & ORC_LEAF_BIT)) {
                 const orc_node* nd = &nodes[cur];
                 kc[0]++;
+                visit_log(q, 0, in_blas ? cur_inst : NULL, nd);
                 uint32_t ref[4], msk[4];
                 int nh = 0;
                 for (int s = 0; s < 4; ++s) {
@@ -460,6 +476,7 @@ static int traverse_pair(qctx* q, v3 o, v3 d0, v3 d1, uint32_t m, float tmin, fl
                 const orc_header* bh = (const orc_header*)(uintptr_t)rec->blas;
                 if (!bh) continue;
                 kc[2]++;
+                visit_log(q, 2, NULL, rec);
                 if (sp >= STACK_MAX) return -1;
                 if (sp >= g_stack_lds) q->cnt.occ_spill_pushes++;
                 stack_ref[sp] = SENTINEL;
@@ -481,6 +498,7 @@ static int traverse_pair(qctx* q, v3 o, v3 d0, v3 d1, uint32_t m, float tmin, fl
             for (uint32_t k = first; k < first + count; ++k) {
                 const orc_tri* tr = &tris[k];
                 kc[1]++;
+                visit_log(q, 1, cur_inst, tr);
                 for (int r = 0; r < 2; ++r) {
                     if (!((mask & alive) >> r & 1u)) continue;
                     float t, u, v;
@@ -1285,4 +1303,37 @@ int orc_trace_image(const rtg_image_job* job, const uint32_t* pixel_list, uint32
         add_counters(counters, &p.total);
     }
     return RTG_OK;
+}
+
+/* Test hook: the visits of one shadow ray's walk (pair = 0: ray d0 alone,
+ * the single-ray occlusion walk) or of the pair walk of d0 and d1 (pair = 1),
+ * with the image's shadow-ray settings (tmin 0.001, tmax 10000,
+ * TerminateOnFirstHit | SkipClosestHit, hit-group offset 1). log receives
+ * cap triples (kind 0 node / 1 triangle / 2 instance entry, the instance
+ * record of the BLAS walked or 0, the record); *n their count. Returns the
+ * walk's result (occluded: 1 / mask of occluded rays) or a negative error. */
+int orc_debug_visits(const void* tlas, const rtg_scene_bindings* scene, const float o[3], const float d0[3],
+                     const float d1[3], int pair, uint64_t* log, uint32_t cap, uint32_t* n) {
+    qctx* q = (qctx*)malloc(sizeof(qctx));
+    if (!q) return RTG_ERROR_OUT_OF_MEMORY;
+    if (qctx_init(q, tlas, scene) != RTG_OK) {
+        free(q);
+        return RTG_ERROR_INVALID_ARGUMENT;
+    }
+    q->ray_flags = RTG_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RTG_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER;
+    q->sbt_offset = 1;
+    q->vlog = log;
+    q->vlog_cap = cap;
+    v3 vo = V3(o[0], o[1], o[2]), va = V3(d0[0], d0[1], d0[2]), vb = V3(d1[0], d1[1], d1[2]);
+    int r;
+    if (pair) {
+        r = traverse_pair(q, vo, va, vb, 3u, 0.001f, 10000.0f);
+    } else {
+        hitrec h;
+        r = traverse(q, vo, va, 0.001f, 10000.0f, 1, &h);
+    }
+    *n = q->vlog_n;
+    int full = q->vlog_n >= cap;
+    free(q);
+    return full ? RTG_ERROR_LIMIT : r;
 }

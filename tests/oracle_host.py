@@ -38,6 +38,7 @@ def lib():
         L.orc_set_stack_lds.restype = None
         L.orc_set_shadow_pairs.argtypes = [C.c_int]
         L.orc_set_shadow_pairs.restype = None
+        L.orc_debug_visits.argtypes = [vp, vp, vp, vp, vp, i32, vp, u32, vp]
         L.orc_trace_image.argtypes = [C.POINTER(abi.ImageJob), vp, u32, i32, C.POINTER(abi.TraceCounters)]
         L.orc_v_smith_ggx_correlated.argtypes = [C.c_float] * 3
         L.orc_v_smith_ggx_correlated.restype = C.c_float

@@ -12,6 +12,8 @@ single-ray query's order, up to its first accepted hit, so:
     holds at least either ray's nodes) and the single-ray count.
 A dropped mask bit, a mask not narrowed at a pop or a hit that ends both
 rays fails one of these."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -60,3 +62,46 @@ def test_pairs_give_the_single_ray_image(scene, nsh):
             assert pa == sa, f
         else:
             assert (sa + 1) // 2 <= pa < sa, f
+
+
+def visits(hs, o, d0, d1, pair):
+    cap = 1 << 16
+    log = np.zeros(3 * cap, dtype=np.uint64)
+    n = np.zeros(1, dtype=np.uint32)
+    f3 = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+    o, d0, d1 = f3(o), f3(d0), f3(d1)
+    r = L.orc_debug_visits(hs.tlas.ptr, C.byref(hs.bindings), o.ctypes.data, d0.ctypes.data, d1.ctypes.data,
+                           int(pair), log.ctypes.data, cap, n.ctypes.data)
+    assert r >= 0
+    return r, [tuple(int(x) for x in log[3 * k:3 * k + 3]) for k in range(int(n[0]))]
+
+
+def test_pair_walk_visits_the_union_of_the_single_walks(scene):
+    """Per ray the pair walk follows the single-ray walk's order up to that
+    ray's first accepted hit, so it visits exactly the union of the two
+    single-ray walks' nodes, triangles and instance entries, each once, and
+    reports each ray occluded as its single walk does. Pairs of shadow rays
+    from random points toward the sun's cone, and pairs of unrelated
+    directions."""
+    s, hs = scene
+    rng = np.random.default_rng(11)
+    sun = np.array(s.uniforms(2).sun_dir[:3], dtype=np.float64)
+    sun /= np.linalg.norm(sun)
+    lo, hi = np.array([-20.0, 0.02, -20.0]), np.array([20.0, 3.0, 20.0])
+    n_pairs, occl_mixed = 300, 0
+    for k in range(n_pairs):
+        o = lo + (hi - lo) * rng.random(3)
+        if k % 3 == 2:
+            d = rng.normal(size=(2, 3))
+        else:
+            d = sun + 0.05 * rng.normal(size=(2, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        r0, v0 = visits(hs, o, d[0], d[1], 0)
+        r1, v1 = visits(hs, o, d[1], d[0], 0)
+        rp, vp = visits(hs, o, d[0], d[1], 1)
+        assert len(set(vp)) == len(vp)
+        assert set(vp) == set(v0) | set(v1)
+        assert rp == (r0 & 1) | ((r1 & 1) << 1)
+        occl_mixed += rp in (1, 2)
+    # some pairs end with one ray occluded and the other not
+    assert occl_mixed > 10
