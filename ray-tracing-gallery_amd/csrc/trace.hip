@@ -317,16 +317,20 @@ __global__ __launch_bounds__(256) void k_raygen(Frame F, bool count) {
     if (count && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long*)&F.counters[0], F.paths);
 }
 
-// Waves of the bounce stream's kernels issue before the other waves of their
-// SIMD (s_setprio: the order in which a SIMD picks among its ready waves,
-// not a queue priority). The bounce levels hold few rays and are bound by
-// the latency of their longest ones; with two frames in flight they and the
-// shadow rays they add form the longest part of a frame's chain (closest
-// hits, shading, bounce levels, resolve, the next frame of that slot), while
-// the first level's shadow rays beside them finish with time to spare. On
-// the 1080p scene the chain's bounce part takes 1.81-1.90 instead of
-// 2.06-2.26 ms per frame.
+// Issue priority of a kernel's waves (s_setprio: the order in which a SIMD
+// picks among its ready waves, not a queue priority). With two frames in
+// flight each frame slot runs a chain: closest hits, shading, the bounce
+// levels with the shadow rays they add, resolve, then the slot's next frame.
+// The first level's shadow rays run beside the bounce levels with time to
+// spare and keep priority 0. The bounce stream's kernels take 2: its levels
+// hold few rays, are bound by the latency of their longest ones and form the
+// longest part of the chain (on the 1080p scene 1.81-1.90 instead of
+// 2.06-2.26 ms per frame). The first level's closest hits and shading take
+// 1: they share the GPU with the other frame's shadow rays, and ahead of
+// them the two frames settled less often into the slower way of overlapping
+// (1 of 16 runs on three boxes, against 3 of 12; the others 0.7 % faster).
 __device__ __forceinline__ void bounce_stream_priority() { __builtin_amdgcn_s_setprio(2); }
+__device__ __forceinline__ void frame_chain_priority() { __builtin_amdgcn_s_setprio(1); }
 
 // Closest-hit traversal of trace level b (count from the device unless fixed).
 // SIDE instances run the bounce levels on the bounce stream. SINGLE instances
@@ -338,6 +342,7 @@ template <bool COUNT, bool SIDE, bool SINGLE>
 __device__ __forceinline__ void trace_closest(Frame F, int level, const uint32_t* count_ptr, uint32_t fixed_count,
                                               uint32_t* status, uint2* s_stack) {
     if (SIDE) bounce_stream_priority();
+    else frame_chain_priority();
     const uint32_t n = min(count_ptr ? *count_ptr : fixed_count, F.path_cap);
     const Queue Q = F.q[level & 1];
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
@@ -420,6 +425,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_closest_rounds(Frame F,
                                                                         uint32_t fixed_count, uint32_t* status) {
     extern __shared__ uint2 s_stack[];
     if (SIDE) bounce_stream_priority();
+    else frame_chain_priority();
     const uint32_t n = min(count_ptr ? *count_ptr : fixed_count, F.path_cap);
     const Queue Q = F.q[level & 1];
     const uint32_t gtid = blockIdx.x * kTraceThreads + threadIdx.x;
@@ -538,6 +544,7 @@ __device__ inline void brdf_parts(V3 n, V3 view, V3 l, float pr, V3 base, float 
 template <bool COUNT, bool ONE>
 __global__ __launch_bounds__(256) void k_shade(Frame F, int level, const uint32_t* count_ptr, uint32_t fixed_count) {
     if (F.side) bounce_stream_priority();
+    else frame_chain_priority();
     const uint32_t n = min(count_ptr ? *count_ptr : fixed_count, F.path_cap);
     const Queue Q = F.q[level & 1];
     const Queue NQ = F.q[(level + 1) & 1];
